@@ -1,0 +1,458 @@
+"""ctypes binding of libgdb_amd.so (include/gdb_amd.h, include/gdb_graph.h).
+
+The shared library is the product: every solver below calls its C ABI and
+runs the HIP kernels.  Nothing here computes a result on the CPU; if the
+library is missing, importing a solver raises instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libgdb_amd.so"
+
+# Return codes (gdb_amd.h).
+GDB_SUCCESS = 0
+GDB_E_INVALID = -1
+GDB_E_HIP = -2
+GDB_E_TIMEOUT = -3
+GDB_E_UNSAFE = -4
+GDB_E_OVERFLOW = -5
+GDB_E_NOMEM = -6
+
+PROTOCOL_LOCKFREE, PROTOCOL_TICKET, PROTOCOL_SPIN = 0, 1, 2
+BARRIER_XF, BARRIER_COUNTER, BARRIER_TREE, BARRIER_AUTO = 0, 1, 2, 3
+SCHED_PERSISTENT, SCHED_RELAUNCH = 0, 1
+VARIANT_PLAIN, VARIANT_TUNED = 0, 1
+
+PANNOTIA_INF = 99999999
+LONESTAR_INF = 1000000000
+
+
+class GdbError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what} failed with code {code}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("wgs", C.c_int),
+        ("groups", C.c_int),
+        ("schedule", C.c_int),
+        ("variant", C.c_int),
+        ("barrier", C.c_int),
+        ("relaunch_check", C.c_int),
+    ]
+
+    @classmethod
+    def make(cls, wgs=256, groups=0, schedule=SCHED_PERSISTENT, variant=VARIANT_PLAIN,
+             barrier=-1, relaunch_check=1) -> "Opts":
+        return cls(wgs, groups, schedule, variant, barrier, relaunch_check)
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("participating", C.c_int),
+        ("rounds", C.c_int),
+        ("barriers", C.c_longlong),
+        ("launches", C.c_longlong),
+        ("kernel_ms", C.c_double),
+        ("work_edges", C.c_longlong),
+        ("work_items", C.c_longlong),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class HostCSRStruct(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("m", C.c_int),
+        ("row", C.POINTER(C.c_int)),
+        ("col", C.POINTER(C.c_int)),
+        ("w", C.POINTER(C.c_int)),
+    ]
+
+
+_lib = None
+P = C.c_void_p
+I = C.c_int
+
+
+def _declare(lib) -> None:
+    sig = {
+        "gdb_malloc": (I, [C.POINTER(P), C.c_size_t]),
+        "gdb_free": (I, [P]),
+        "gdb_memcpy_htod": (I, [P, P, C.c_size_t]),
+        "gdb_memcpy_dtoh": (I, [P, P, C.c_size_t]),
+        "gdb_memcpy_dtod": (I, [P, P, C.c_size_t]),
+        "gdb_device_synchronize": (I, []),
+        "gdb_device_count": (I, [C.POINTER(I)]),
+        "gdb_ctx_create": (I, [C.POINTER(P)]),
+        "gdb_ctx_destroy": (I, [P]),
+        "gdb_ctx_configure": (I, [P, I, I]),
+        "gdb_ctx_set_timeouts": (I, [P, C.c_double, C.c_double]),
+        "gdb_ctx_set_xcd_only": (I, [P, I]),
+        "gdb_ctx_init": (I, [P]),
+        "gdb_ctx_init_skip": (I, [P, I]),
+        "gdb_number_of_participating_groups": (I, [P]),
+        "gdb_ctx_device_error": (I, [P]),
+        "gdb_ctx_stream": (P, [P]),
+        "gdb_time_protocol": (I, [P, I, I, I, C.POINTER(C.c_double)]),
+        "gdb_occupancy_test": (I, [P, I, I, I, I, C.POINTER(I)]),
+        "gdb_max_resident_groups": (I, [I, I, C.POINTER(I)]),
+        "gdb_device_query": (I, [C.c_char_p, C.c_size_t]),
+        "gdb_barrier_bench": (I, [P, I, I, I, C.POINTER(C.c_double), C.POINTER(I)]),
+        "gdb_barrier_check": (I, [P, I, I, I, C.POINTER(C.c_longlong), C.POINTER(I)]),
+        "gdb_relaunch_bench": (I, [P, I, I, I, I, C.POINTER(C.c_double)]),
+        "gdb_color": (I, [P, I, I, P, P, P, P, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_csr_free": (None, [C.POINTER(HostCSRStruct)]),
+        "gdb_gen_rmat": (I, [I, I, C.c_double, C.c_double, C.c_double, C.c_uint64, I, I,
+                             C.POINTER(HostCSRStruct)]),
+        "gdb_gen_grid": (I, [I, I, C.c_double, C.c_uint64, I, I, C.POINTER(HostCSRStruct)]),
+        "gdb_gen_gnp": (I, [I, C.c_double, C.c_uint64, I, I, C.POINTER(HostCSRStruct)]),
+        "gdb_gen_permutation": (I, [I, C.c_uint64, P]),
+        "gdb_gen_uniform_ints": (I, [C.c_longlong, I, I, C.c_uint64, P]),
+        "gdb_csr_transpose": (I, [C.POINTER(HostCSRStruct), C.POINTER(HostCSRStruct)]),
+    }
+    optional = {
+        "gdb_mis": (I, [P, I, I, P, P, P, P, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_sssp_pull": (I, [P, I, I, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_bc": (I, [P, I, I, P, P, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_bfs_worklist": (I, [P, I, I, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_sssp_worklist": (I, [P, I, I, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_verify_distances": (I, [P, I, I, P, P, P, P, C.POINTER(C.c_longlong)]),
+    }
+    for name, (res, args) in {**sig, **optional}.items():
+        if name in optional and not hasattr(lib, name):
+            continue
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """Loads libgdb_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m gpu_discovery_barrier_amd.build`")
+        _lib = C.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != GDB_SUCCESS:
+        raise GdbError(rc, what)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().gdb_device_count(C.byref(n)), "gdb_device_count")
+    return n.value
+
+
+# ---------------------------------------------------------------------------
+# Device memory
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """A device allocation made by gdb_malloc."""
+
+    def __init__(self, nbytes: int, dtype=np.int32, count: int | None = None):
+        self.ptr = P()
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.count = count if count is not None else self.nbytes // self.dtype.itemsize
+        check(lib().gdb_malloc(C.byref(self.ptr), max(self.nbytes, 4)), "gdb_malloc")
+
+    @classmethod
+    def empty(cls, count: int, dtype=np.int32) -> "DeviceBuffer":
+        return cls(int(count) * np.dtype(dtype).itemsize, dtype, int(count))
+
+    @classmethod
+    def from_numpy(cls, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes, arr.dtype, arr.size)
+        if arr.nbytes:
+            check(lib().gdb_memcpy_htod(buf.ptr, arr.ctypes.data_as(P), arr.nbytes), "gdb_memcpy_htod")
+        return buf
+
+    def to_numpy(self) -> np.ndarray:
+        out = np.empty(self.count, dtype=self.dtype)
+        if out.nbytes:
+            check(lib().gdb_memcpy_dtoh(out.ctypes.data_as(P), self.ptr, out.nbytes), "gdb_memcpy_dtoh")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().gdb_free(self.ptr)
+            self.ptr = P()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Host graphs (generators run in the library's host code)
+# ---------------------------------------------------------------------------
+@dataclass
+class CSR:
+    n: int
+    row: np.ndarray  # int32[n+1]
+    col: np.ndarray  # int32[m]
+    w: np.ndarray    # int32[m]
+
+    @property
+    def m(self) -> int:
+        return int(self.col.size)
+
+    def transpose(self) -> "CSR":
+        src = _to_struct(self)
+        out = HostCSRStruct()
+        check(lib().gdb_csr_transpose(C.byref(src), C.byref(out)), "gdb_csr_transpose")
+        return _from_struct(out)
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.row)
+
+
+def _from_struct(s: HostCSRStruct) -> CSR:
+    n, m = s.n, s.m
+    row = np.ctypeslib.as_array(s.row, shape=(n + 1,)).copy()
+    col = np.ctypeslib.as_array(s.col, shape=(max(m, 1),))[:m].copy()
+    w = np.ctypeslib.as_array(s.w, shape=(max(m, 1),))[:m].copy()
+    lib().gdb_csr_free(C.byref(s))
+    return CSR(n, row, col, w)
+
+
+def _to_struct(g: CSR) -> HostCSRStruct:
+    s = HostCSRStruct()
+    s.n, s.m = g.n, g.m
+    g._keep = [np.ascontiguousarray(g.row, np.int32), np.ascontiguousarray(g.col, np.int32),
+               np.ascontiguousarray(g.w, np.int32)]
+    s.row = g._keep[0].ctypes.data_as(C.POINTER(C.c_int))
+    s.col = g._keep[1].ctypes.data_as(C.POINTER(C.c_int))
+    s.w = g._keep[2].ctypes.data_as(C.POINTER(C.c_int))
+    return s
+
+
+def gen_rmat(scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1,
+             wlo: int = 1, whi: int = 1) -> CSR:
+    out = HostCSRStruct()
+    check(lib().gdb_gen_rmat(scale, edge_factor, a, b, c, seed, wlo, whi, C.byref(out)), "gdb_gen_rmat")
+    return _from_struct(out)
+
+
+def gen_grid(rows: int, cols: int, remove_frac: float = 0.0, seed: int = 2, wlo: int = 1,
+             whi: int = 1) -> CSR:
+    out = HostCSRStruct()
+    check(lib().gdb_gen_grid(rows, cols, remove_frac, seed, wlo, whi, C.byref(out)), "gdb_gen_grid")
+    return _from_struct(out)
+
+
+def gen_gnp(n: int, avg_degree: float = 8.0, seed: int = 4, wlo: int = 1, whi: int = 1) -> CSR:
+    out = HostCSRStruct()
+    check(lib().gdb_gen_gnp(n, avg_degree, seed, wlo, whi, C.byref(out)), "gdb_gen_gnp")
+    return _from_struct(out)
+
+
+def permutation(n: int, seed: int) -> np.ndarray:
+    out = np.empty(n, np.int32)
+    check(lib().gdb_gen_permutation(n, seed, out.ctypes.data_as(P)), "gdb_gen_permutation")
+    return out
+
+
+def uniform_ints(count: int, lo: int, hi: int, seed: int) -> np.ndarray:
+    out = np.empty(count, np.int32)
+    check(lib().gdb_gen_uniform_ints(count, lo, hi, seed, out.ctypes.data_as(P)), "gdb_gen_uniform_ints")
+    return out
+
+
+def pannotia_values(n: int) -> np.ndarray:
+    """node_value[i] = i / (float)(n + 1) (color_gb.cpp:152, mis_gb.cpp:151)."""
+    return (np.arange(n, dtype=np.float32) / np.float32(n + 1)).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Device graphs and the discovery context
+# ---------------------------------------------------------------------------
+class DeviceCSR:
+    def __init__(self, g: CSR, weights: bool = True):
+        self.n, self.m = g.n, g.m
+        self.row = DeviceBuffer.from_numpy(g.row.astype(np.int32))
+        self.col = DeviceBuffer.from_numpy(g.col.astype(np.int32))
+        self.w = DeviceBuffer.from_numpy(g.w.astype(np.int32)) if weights else None
+
+
+class Context:
+    """A discovery context (gdb_ctx): device protocol/barrier state + stream."""
+
+    def __init__(self, lock_kind: int = PROTOCOL_LOCKFREE, bar_kind: int = BARRIER_AUTO):
+        self.h = P()
+        check(lib().gdb_ctx_create(C.byref(self.h)), "gdb_ctx_create")
+        self.configure(lock_kind, bar_kind)
+
+    def configure(self, lock_kind: int, bar_kind: int) -> None:
+        check(lib().gdb_ctx_configure(self.h, lock_kind, bar_kind), "gdb_ctx_configure")
+        check(lib().gdb_ctx_init(self.h), "gdb_ctx_init")
+
+    def set_xcd_only(self, xcd: int) -> None:
+        check(lib().gdb_ctx_set_xcd_only(self.h, xcd), "gdb_ctx_set_xcd_only")
+        check(lib().gdb_ctx_init(self.h), "gdb_ctx_init")
+
+    def set_timeouts(self, wait_us: float, quiet_us: float) -> None:
+        check(lib().gdb_ctx_set_timeouts(self.h, wait_us, quiet_us), "gdb_ctx_set_timeouts")
+
+    def init(self, skip: bool = False) -> None:
+        check(lib().gdb_ctx_init_skip(self.h, int(skip)), "gdb_ctx_init_skip")
+
+    def participating(self) -> int:
+        rc = lib().gdb_number_of_participating_groups(self.h)
+        if rc < 0:
+            raise GdbError(rc, "gdb_number_of_participating_groups")
+        return rc
+
+    def close(self) -> None:
+        if self.h:
+            lib().gdb_ctx_destroy(self.h)
+            self.h = P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- protocol / barrier tools ---------------------------------------
+    def time_protocol(self, upper_bound: int, wgs: int, lms: int = 0) -> float:
+        ns = C.c_double(0)
+        check(lib().gdb_time_protocol(self.h, upper_bound, wgs, lms, C.byref(ns)), "gdb_time_protocol")
+        return ns.value
+
+    def occupancy_test(self, wgc: int, wgs: int, lms: int = 0, prot: bool = True) -> int:
+        p = C.c_int(0)
+        check(lib().gdb_occupancy_test(self.h, wgc, wgs, lms, int(prot), C.byref(p)), "gdb_occupancy_test")
+        return p.value
+
+    def barrier_bench(self, wgs: int, groups: int, iters: int) -> tuple[float, int]:
+        ms, p = C.c_double(0), C.c_int(0)
+        check(lib().gdb_barrier_bench(self.h, wgs, groups, iters, C.byref(ms), C.byref(p)),
+              "gdb_barrier_bench")
+        return ms.value, p.value
+
+    def barrier_check(self, wgs: int, groups: int, iters: int) -> tuple[int, int]:
+        bad, p = C.c_longlong(0), C.c_int(0)
+        check(lib().gdb_barrier_check(self.h, wgs, groups, iters, C.byref(bad), C.byref(p)),
+              "gdb_barrier_check")
+        return bad.value, p.value
+
+    def relaunch_bench(self, wgs: int, groups: int, iters: int, mode: int = 0) -> float:
+        ms = C.c_double(0)
+        check(lib().gdb_relaunch_bench(self.h, wgs, groups, iters, mode, C.byref(ms)), "gdb_relaunch_bench")
+        return ms.value
+
+
+def max_resident_groups(wgs: int, lms: int = 0) -> int:
+    out = C.c_int(0)
+    check(lib().gdb_max_resident_groups(wgs, lms, C.byref(out)), "gdb_max_resident_groups")
+    return out.value
+
+
+def device_query() -> str:
+    buf = C.create_string_buffer(4096)
+    check(lib().gdb_device_query(buf, 4096), "gdb_device_query")
+    return buf.value.decode()
+
+
+# ---------------------------------------------------------------------------
+# Solvers
+# ---------------------------------------------------------------------------
+def _opts(opts: Opts | dict | None) -> Opts:
+    if opts is None:
+        return Opts.make()
+    if isinstance(opts, dict):
+        return Opts.make(**opts)
+    return opts
+
+
+def color(ctx: Context, g: DeviceCSR, node_value: DeviceBuffer, opts=None, out: DeviceBuffer | None = None):
+    out = out or DeviceBuffer.empty(g.n, np.int32)
+    st = Stats()
+    o = _opts(opts)
+    check(lib().gdb_color(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr, node_value.ptr, out.ptr, C.byref(o),
+                          C.byref(st)), "gdb_color")
+    return out.to_numpy(), st
+
+
+def mis(ctx: Context, g: DeviceCSR, node_value: DeviceBuffer, opts=None, out: DeviceBuffer | None = None):
+    out = out or DeviceBuffer.empty(g.n, np.int32)
+    st = Stats()
+    o = _opts(opts)
+    check(lib().gdb_mis(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr, node_value.ptr, out.ptr, C.byref(o),
+                        C.byref(st)), "gdb_mis")
+    return out.to_numpy(), st
+
+
+def sssp_pull(ctx: Context, gt: DeviceCSR, source: int = 0, opts=None, out: DeviceBuffer | None = None):
+    out = out or DeviceBuffer.empty(gt.n, np.int32)
+    st = Stats()
+    o = _opts(opts)
+    check(lib().gdb_sssp_pull(ctx.h, gt.n, gt.m, gt.row.ptr, gt.col.ptr, gt.w.ptr, source, out.ptr,
+                              C.byref(o), C.byref(st)), "gdb_sssp_pull")
+    return out.to_numpy(), st
+
+
+def bc(ctx: Context, g: DeviceCSR, gt: DeviceCSR, sources: np.ndarray, opts=None,
+       out: DeviceBuffer | None = None):
+    out = out or DeviceBuffer.empty(g.n, np.float32)
+    src = DeviceBuffer.from_numpy(np.asarray(sources, np.int32))
+    st = Stats()
+    o = _opts(opts)
+    check(lib().gdb_bc(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr, gt.row.ptr, gt.col.ptr, src.ptr,
+                       int(len(sources)), out.ptr, C.byref(o), C.byref(st)), "gdb_bc")
+    return out.to_numpy(), st
+
+
+def bfs_worklist(ctx: Context, g: DeviceCSR, source: int = 0, opts=None, out: DeviceBuffer | None = None):
+    out = out or DeviceBuffer.empty(g.n, np.uint32)
+    st = Stats()
+    o = _opts(opts)
+    check(lib().gdb_bfs_worklist(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr, source, out.ptr, C.byref(o),
+                                 C.byref(st)), "gdb_bfs_worklist")
+    return out.to_numpy(), st
+
+
+def sssp_worklist(ctx: Context, g: DeviceCSR, source: int = 0, opts=None,
+                  out: DeviceBuffer | None = None):
+    out = out or DeviceBuffer.empty(g.n, np.uint32)
+    st = Stats()
+    o = _opts(opts)
+    check(lib().gdb_sssp_worklist(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr, g.w.ptr, source, out.ptr,
+                                  C.byref(o), C.byref(st)), "gdb_sssp_worklist")
+    return out.to_numpy(), st
+
+
+def verify_distances(ctx: Context, g: DeviceCSR, dist: DeviceBuffer, weighted: bool) -> int:
+    nerr = C.c_longlong(0)
+    check(lib().gdb_verify_distances(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr,
+                                     g.w.ptr if weighted else None, dist.ptr, C.byref(nerr)),
+          "gdb_verify_distances")
+    return nerr.value

@@ -1,0 +1,296 @@
+// Pannotia graph colouring (max rule) on the discovery protocol.
+//
+// Rule (code/experiments/pannotia/graph_app/color/kernel/color_kernel.cl):
+// round r = 1, 2, ...: for each uncoloured v, max_d[v] = max node_value[u]
+// over neighbours u that are uncoloured, starting from -1, with every
+// neighbour ignored when deg(v) == 1 (the `start != end - 1` test, lines
+// 112/204); `stop` is raised when such a neighbour exists.  Then every
+// uncoloured v with node_value[v] > max_d[v] takes colour r.  The loop ends
+// after the first round whose stop flag stayed 0 (lines 240-242).
+//
+// Schedules:
+//   plain persistent  - the mega_kernel structure (lines 159-252): two phases
+//                       and two barriers per round, thread per vertex.
+//   tuned persistent  - one barrier per round: phase r also applies round
+//                       r-1's colouring (v coloured in r-1 iff uncoloured and
+//                       node_value[v] > max_d_{r-1}[v]), so a neighbour's state
+//                       is read as "uncoloured after r-1" from color[] and the
+//                       previous max_d buffer; high-degree vertices are walked
+//                       by a whole wave.
+//   relaunch          - color / color2 kernels per round with a host loop
+//                       (color.cpp:294-334).
+#include "graph_common.hpp"
+#include "runtime.hpp"
+
+namespace {
+
+constexpr int kSerialLimit = 16;
+
+template <int WGS>
+__global__ __launch_bounds__(WGS) void k_color_plain(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
+                                                     const int* __restrict__ col,
+                                                     const float* __restrict__ nv, int* color,
+                                                     int* stop1, int* stop2, float* max_d, int n,
+                                                     int max_rounds, int* out) {
+  DISCOVERY_PROTOCOL(ctx);
+  int* write_stop = stop1;
+  int* read_stop = stop2;
+  const int tid = p_get_global_id(ctx, &local_ctx);
+  const int stride = p_get_global_size(ctx, &local_ctx);
+  int graph_color = 1;
+  int nbar = 0;
+  for (;;) {
+    bool st = false;
+    for (int i = tid; i < n; i += stride) {
+      if (color[i] == -1) {
+        const int start = row[i], end = row[i + 1];
+        float mx = -1.f;
+        for (int e = start; e < end; ++e) {
+          const int u = col[e];
+          if (color[u] == -1 && start != end - 1) {
+            st = true;
+            const float x = nv[u];
+            if (x > mx) mx = x;
+          }
+        }
+        max_d[i] = mx;
+      }
+    }
+    if (st) gdb::st_rlx(write_stop, 1);
+    int* sw = read_stop;
+    read_stop = write_stop;
+    write_stop = sw;
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    for (int i = tid; i < n; i += stride)
+      if (color[i] == -1 && nv[i] > max_d[i]) color[i] = graph_color;
+    if (gdb::ld_rlx(read_stop) == 0) break;
+    if (graph_color >= max_rounds) {
+      gdb::set_error(ctx, GDB_ERR_QUEUE_OVERFLOW);
+      break;
+    }
+    ++graph_color;
+    gdb::st_rlx(write_stop, 0);
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+  }
+  if (tid == 0) {
+    out[0] = graph_color;
+    out[1] = nbar;
+  }
+}
+
+template <int WGS>
+__global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
+                                                     const int* __restrict__ col,
+                                                     const float* __restrict__ nv, int* color,
+                                                     int* stops, float* maxd, int n, int max_rounds,
+                                                     int* out) {
+  DISCOVERY_PROTOCOL(ctx);
+  const int tid = p_get_global_id(ctx, &local_ctx);
+  const int stride = p_get_global_size(ctx, &local_ctx);
+  int r = 1;
+  int nbar = 0;
+  for (;;) {
+    const float* md_prev = maxd + (size_t)((r - 1) & 1) * n;
+    float* md_cur = maxd + (size_t)(r & 1) * n;
+    const bool have_prev = r > 1;
+    if (tid == 0) gdb::st_rlx(&stops[(r + 1) % 3], 0);
+    bool st = false;
+    GDB_FOR_UNIFORM(v, valid, tid, stride, n) {
+      bool active = false;
+      int b = 0, e = 0;
+      if (valid && color[v] == -1) {
+        if (have_prev && nv[v] > md_prev[v]) {
+          color[v] = r - 1;
+        } else {
+          active = true;
+          b = row[v];
+          e = row[v + 1];
+        }
+      }
+      const bool deg1 = (e - b) == 1;
+      auto edge = [&](int k) -> gdb::MaxFound {
+        const int u = col[k];
+        const bool unc = color[u] == -1 && !(have_prev && nv[u] > md_prev[u]);
+        return unc ? gdb::MaxFound{nv[u], 1} : gdb::MaxFound{-1.f, 0};
+      };
+      auto red = [](gdb::MaxFound a, gdb::MaxFound c) {
+        return gdb::MaxFound{a.mx > c.mx ? a.mx : c.mx, a.found | c.found};
+      };
+      const gdb::MaxFound res =
+          gdb::wave_coop_reduce(active && !deg1, b, e, gdb::MaxFound{-1.f, 0}, edge, red, kSerialLimit);
+      if (active) {
+        md_cur[v] = deg1 ? -1.f : res.mx;
+        if (!deg1 && res.found) st = true;
+      }
+    }
+    if (gdb::wave_any(st) && gdb::lane_id() == 0) gdb::st_rlx(&stops[r % 3], 1);
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    if (gdb::ld_rlx(&stops[r % 3]) == 0) break;
+    if (r >= max_rounds) {
+      gdb::set_error(ctx, GDB_ERR_QUEUE_OVERFLOW);
+      break;
+    }
+    ++r;
+  }
+  // Round r's own colouring (the reference's color2 of the last round).
+  const float* md = maxd + (size_t)(r & 1) * n;
+  for (int v = tid; v < n; v += stride)
+    if (color[v] == -1 && nv[v] > md[v]) color[v] = r;
+  if (tid == 0) {
+    out[0] = r;
+    out[1] = nbar;
+  }
+}
+
+// ---- relaunch schedule: color (color_kernel.cl:80-123) and color2 (134-154)
+// `flags` = {stop[0..2], done, last}.  Round r writes stop[r % 3]; a kernel of
+// round r is a no-op once round r-1 ended with stop 0 (done is sticky).
+__global__ void k_color_r1(const int* __restrict__ row, const int* __restrict__ col,
+                           const float* __restrict__ nv, const int* color, int* flags,
+                           float* max_d, int n, int r) {
+  if (flags[3] || (r > 1 && flags[(r - 1) % 3] == 0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !flags[3]) {
+      flags[3] = 1;
+      flags[4] = r - 1;
+    }
+    return;
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && color[i] == -1) {
+    const int start = row[i], end = row[i + 1];
+    float mx = -1.f;
+    bool st = false;
+    for (int e = start; e < end; ++e) {
+      const int u = col[e];
+      if (color[u] == -1 && start != end - 1) {
+        st = true;
+        const float x = nv[u];
+        if (x > mx) mx = x;
+      }
+    }
+    max_d[i] = mx;
+    if (st) flags[r % 3] = 1;
+  }
+}
+
+__global__ void k_color_r2(const float* __restrict__ nv, int* color, int* flags,
+                           const float* max_d, int n, int r) {
+  if (flags[3] || (r > 1 && flags[(r - 1) % 3] == 0)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && color[i] == -1 && nv[i] > max_d[i]) color[i] = r;
+  if (i == 0) flags[(r + 1) % 3] = 0;
+}
+
+__global__ void k_color_init(int* color, float* max_d, int* flags, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    color[i] = -1;
+    max_d[i] = -1.f;
+    max_d[i + n] = -1.f;
+  }
+  if (i < 8) flags[i] = 0;
+}
+
+}  // namespace
+
+using namespace gdbrt;
+
+extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* col,
+                         const float* node_value, int* color, const gdb_opts* opts,
+                         gdb_stats* stats) {
+  if (!ctx || n < 0 || m < 0 || (n > 0 && (!row || !node_value || !color)) || (m > 0 && !col))
+    return GDB_E_INVALID;
+  gdb_opts o{};
+  if (opts) o = *opts;
+  const int wgs = default_wgs(&o);
+  gdb_stats s{};
+  if (n == 0) {
+    if (stats) *stats = s;
+    return GDB_SUCCESS;
+  }
+  const size_t bytes = carve_bytes<float>((size_t)2 * n) + carve_bytes<int>(16);
+  int rc = ensure_workspace(ctx, bytes);
+  if (rc) return rc;
+  Carve cv(ctx->ws);
+  float* maxd = cv.take<float>((size_t)2 * n);
+  int* flags = cv.take<int>(16);
+  const int max_rounds = n + 1;
+  const int init_groups = (n + 255) / 256;
+  hipLaunchKernelGGL(k_color_init, dim3(init_groups), dim3(256), 0, ctx->stream, color, maxd, flags, n);
+  GDB_HIP(hipGetLastError());
+  rc = prepare_launch(ctx);
+  if (rc) return rc;
+
+  if (o.schedule == GDB_SCHED_RELAUNCH) {
+    const int grid = (n + wgs - 1) / wgs;
+    const int check = o.relaunch_check > 1 ? o.relaunch_check : 1;
+    rc = timer_start(ctx);
+    if (rc) return rc;
+    int r = 1, last = 0;
+    for (;;) {
+      for (int k = 0; k < check; ++k, ++r) {
+        hipLaunchKernelGGL(k_color_r1, dim3(grid), dim3(wgs), 0, ctx->stream, row, col, node_value,
+                           color, flags, maxd, n, r);
+        hipLaunchKernelGGL(k_color_r2, dim3(grid), dim3(wgs), 0, ctx->stream, node_value, color,
+                           flags, maxd, n, r);
+        s.launches += 2;
+      }
+      GDB_HIP(hipGetLastError());
+      GDB_HIP(hipMemcpyAsync(ctx->h_flag, flags, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      GDB_HIP(hipStreamSynchronize(ctx->stream));
+      const int lr = r - 1;  // last launched round
+      if (ctx->h_flag[3]) {
+        int tail[2];
+        GDB_HIP(hipMemcpy(tail, flags + 3, 2 * sizeof(int), hipMemcpyDeviceToHost));
+        last = tail[1];
+        break;
+      }
+      if (ctx->h_flag[lr % 3] == 0) {
+        last = lr;
+        break;
+      }
+      if (lr >= max_rounds) return GDB_E_TIMEOUT;
+    }
+    rc = timer_stop(ctx, &s.kernel_ms);
+    if (rc) return rc;
+    s.rounds = last;
+    s.participating = grid;
+  } else {
+    const bool tuned = o.variant == GDB_VARIANT_TUNED;
+    int* out = flags + 8;
+    rc = gdb_dispatch_wgs(wgs, [&](auto W) -> int {
+      constexpr int WG = decltype(W)::value;
+      const void* kern = tuned ? (const void*)k_color_tuned<WG> : (const void*)k_color_plain<WG>;
+      int groups = 0;
+      int r2 = launch_groups(ctx, kern, &o, n, &groups);
+      if (r2) return r2;
+      r2 = timer_start(ctx);
+      if (r2) return r2;
+      if (tuned)
+        hipLaunchKernelGGL(k_color_tuned<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row,
+                           col, node_value, color, flags, maxd, n, max_rounds, out);
+      else
+        hipLaunchKernelGGL(k_color_plain<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row,
+                           col, node_value, color, flags, flags + 4, maxd, n, max_rounds, out);
+      GDB_HIP(hipGetLastError());
+      return timer_stop(ctx, &s.kernel_ms);
+    });
+    if (rc) return rc;
+    s.launches = 1;
+    int hout[2] = {0, 0};
+    GDB_HIP(hipMemcpy(hout, out, sizeof(hout), hipMemcpyDeviceToHost));
+    s.rounds = hout[0];
+    s.barriers = hout[1];
+    rc = read_participating(ctx, &s.participating);
+    if (rc) return rc;
+  }
+  rc = check_device_error(ctx);
+  if (rc == GDB_E_OVERFLOW) rc = GDB_E_TIMEOUT;  // round cap hit: equal priorities never resolve
+  s.work_edges = m;
+  s.work_items = n;
+  if (stats) *stats = s;
+  return rc;
+}

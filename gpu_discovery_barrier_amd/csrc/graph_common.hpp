@@ -1,0 +1,63 @@
+// Wave64 helpers shared by the graph kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "discovery.hpp"
+
+namespace gdb {
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ float shfl_xor(float x, int o) { return __shfl_xor(x, o, 64); }
+__device__ __forceinline__ int shfl_xor(int x, int o) { return __shfl_xor(x, o, 64); }
+
+struct MaxFound {  // running max plus "at least one candidate seen"
+  float mx;
+  int found;
+};
+__device__ __forceinline__ MaxFound shfl_xor(MaxFound x, int o) {
+  return MaxFound{__shfl_xor(x.mx, o, 64), __shfl_xor(x.found, o, 64)};
+}
+
+// Thread / wave cooperative reduction over the edge ranges [b, e) held by
+// the lanes of one wave.  Lanes whose range is at most `serial_limit` edges
+// walk it alone; longer ranges are walked by the whole wave (64 consecutive
+// edges per step, coalesced) and reduced with xor-shuffles.  `edge(k)` maps
+// an edge index to a value, `red` combines two values.  All 64 lanes must
+// call this (wave-uniform control flow).
+template <typename T, typename EdgeFn, typename RedFn>
+__device__ __forceinline__ T wave_coop_reduce(bool active, int b, int e, T identity, EdgeFn edge,
+                                              RedFn red, int serial_limit) {
+  T acc = identity;
+  const bool heavy = active && (e - b) > serial_limit;
+  if (active && !heavy)
+    for (int k = b; k < e; ++k) acc = red(acc, edge(k));
+  unsigned long long mask = __ballot(heavy);
+  const int lane = lane_id();
+  while (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    const int lb = __shfl(b, leader, 64);
+    const int le = __shfl(e, leader, 64);
+    T part = identity;
+    for (int k = lb + lane; k < le; k += 64) part = red(part, edge(k));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part = red(part, shfl_xor(part, off));
+    if (lane == leader) acc = part;
+    mask &= mask - 1;
+  }
+  return acc;
+}
+
+// Wave-wide OR of a predicate.
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// Uniform-trip grid-stride loop over [0, n): every thread of a group runs
+// the same number of iterations (i may be >= n in the last one), so the body
+// may use wave- and group-wide collectives.
+#define GDB_FOR_UNIFORM(i, valid, start, stride, n)                             \
+  for (long long _base = (long long)(start) - (long long)threadIdx.x;           \
+       _base < (long long)(n); _base += (long long)(stride))                      \
+    for (int _once = 1, i = (int)(_base + threadIdx.x),                          \
+             valid = (_base + (long long)threadIdx.x) < (long long)(n);          \
+         _once; _once = 0)
+
+}  // namespace gdb

@@ -1,0 +1,179 @@
+/* C ABI of libgdb_amd.so: occupancy discovery, inter-workgroup barriers and
+ * the persistent graph kernels built on them, for AMD Instinct MI355X
+ * (gfx950).
+ *
+ * Every entry point returns an int: GDB_SUCCESS (0) or a negative GDB_E_*
+ * code.  Graph arrays passed to the solvers are DEVICE pointers (use
+ * gdb_malloc / gdb_memcpy_htod, or any HIP allocation); all work runs on the
+ * context's stream and the call returns after it finished.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   gdb_ctx_create + gdb_ctx_init
+ *        clCreateBuffer(sizeof(discovery_kernel_ctx)) + init_discovery_kernel_ctx
+ *        (code/discovery_protocol/api/discovery.h:10-32; e.g.
+ *         code/experiments/pannotia/graph_app/color/color_gb.cpp:182-185)
+ *   gdb_ctx_init_skip          init_discovery_kernel_ctx_skip  (discovery.h:36-59)
+ *   gdb_number_of_participating_groups
+ *                              number_of_participating_groups  (discovery.h:63-68)
+ *   gdb_time_protocol          time_protocol                   (discovery.h:71-105)
+ *   gdb_occupancy_test         occupancy_test main + run_test kernel
+ *        (code/experiments/occupancy_tests/src/occupancy_test.c:21-89,
+ *         kernels/occupancy_test.cl:8-11)
+ *   gdb_device_query           device_query / print_device_info
+ *        (occupancy_tests/src/device_query.c; common/include/OpenCL/my_opencl.h:193-217)
+ *   gdb_color                  color-gb mega_kernel / color relaunch loop
+ *        (pannotia/graph_app/color/kernel/color_kernel.cl:80-252; color.cpp:294-334)
+ *   gdb_mis                    mis-gb mega_kernel / mis relaunch loop
+ *        (pannotia/graph_app/mis/kernel/mis_kernel.cl:78-337; mis.cpp:314-363)
+ *   gdb_sssp_pull              sssp-gb mega_kernel / sssp relaunch loop
+ *        (pannotia/graph_app/sssp/kernel/sssp_kernel.cl:79-244; sssp.cpp:290-344)
+ *   gdb_bc                     bc-gb mega_bc_kernel / bc relaunch loop
+ *        (pannotia/graph_app/bc/kernel/bc_kernel.cl:95-513; bc.cpp:322-419)
+ *   gdb_bfs_worklist           bfs-port drelax2 + dverifysolution
+ *        (lonestar_gpu_opencl/apps/bfs/kernel/bfs_port_kernel.cl:35-192,
+ *         bfs_common.cl:43-90; bfs_port.cpp:62-215)
+ *   gdb_sssp_worklist          sssp-port drelax2 + dverifysolution
+ *        (lonestar_gpu_opencl/apps/sssp/kernel/sssp_port_kernel.cl:42-197,
+ *         sssp_common.cl:41-81; sssp_port.cpp:65-178)
+ */
+#ifndef GDB_AMD_H
+#define GDB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---------------------------------------------------- */
+#define GDB_SUCCESS 0
+#define GDB_E_INVALID (-1)   /* bad argument                                   */
+#define GDB_E_HIP (-2)       /* HIP runtime error (message printed to stderr)  */
+#define GDB_E_TIMEOUT (-3)   /* a device-side wait hit its deadline            */
+#define GDB_E_UNSAFE (-4)    /* skip-mode launch larger than the residency bound */
+#define GDB_E_OVERFLOW (-5)  /* a device queue overflowed                      */
+#define GDB_E_NOMEM (-6)     /* device allocation failed                       */
+
+/* ---- protocol / barrier / schedule selectors ---------------------------- */
+#define GDB_PROTOCOL_LOCKFREE 0 /* single atomic ticket word (default)        */
+#define GDB_PROTOCOL_TICKET 1   /* fair ticket lock, locks.cl:25-34           */
+#define GDB_PROTOCOL_SPIN 2     /* unfair spin lock, locks.cl:11-20           */
+
+#define GDB_BARRIER_XF 0       /* master/slave flags, discovery.cl:129-190    */
+#define GDB_BARRIER_COUNTER 1  /* central sense-reversing counter             */
+#define GDB_BARRIER_TREE 2     /* per-XCD arrival, one L2 write-back per XCD  */
+#define GDB_BARRIER_AUTO 3     /* COUNTER below 64 groups, TREE above (default) */
+
+#define GDB_SCHED_PERSISTENT 0 /* one launch, inter-workgroup barriers        */
+#define GDB_SCHED_RELAUNCH 1   /* one launch per phase, host loop             */
+
+#define GDB_VARIANT_PLAIN 0    /* phase structure of the reference kernels    */
+#define GDB_VARIANT_TUNED 1    /* gfx950 kernels: frontier queues, wave-level work split */
+
+typedef struct gdb_ctx gdb_ctx;
+
+typedef struct {
+  int wgs;            /* threads per workgroup: 64,128,256,512,1024 (0 = 256) */
+  int groups;         /* groups launched (0 = device residency bound)          */
+  int schedule;       /* GDB_SCHED_*                                           */
+  int variant;        /* GDB_VARIANT_*                                         */
+  int barrier;        /* GDB_BARRIER_* or -1 for the context default           */
+  int relaunch_check; /* relaunch schedule: rounds between host stop checks (0/1 = every round) */
+} gdb_opts;
+
+typedef struct {
+  int participating;     /* groups that took part (discovered P)              */
+  int rounds;            /* rounds / levels / iterations executed             */
+  long long barriers;    /* inter-workgroup barriers executed (per group)     */
+  long long launches;    /* kernel launches in the solve                      */
+  double kernel_ms;      /* solve time, device events around the kernels      */
+  long long work_edges;  /* edges examined (for algorithmic-byte accounting)  */
+  long long work_items;  /* vertices / worklist items processed               */
+} gdb_stats;
+
+/* ---- device memory helpers ------------------------------------------------ */
+int gdb_malloc(void** ptr, size_t bytes);
+int gdb_free(void* ptr);
+int gdb_memcpy_htod(void* dst, const void* src, size_t bytes);
+int gdb_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int gdb_memcpy_dtod(void* dst, const void* src, size_t bytes);
+int gdb_device_synchronize(void);
+int gdb_device_count(int* out);
+
+/* ---- discovery context ------------------------------------------------------ */
+int gdb_ctx_create(gdb_ctx** out);
+int gdb_ctx_destroy(gdb_ctx* ctx);
+/* lock_kind: GDB_PROTOCOL_*, bar_kind: GDB_BARRIER_*; takes effect at next init. */
+int gdb_ctx_configure(gdb_ctx* ctx, int lock_kind, int bar_kind);
+/* Restricts participation to the groups running on one XCD (0..7), or
+ * lifts the restriction (-1, default); takes effect at next init.  Such
+ * launches synchronise through that XCD's shared L2 (cheapest barrier). */
+int gdb_ctx_set_xcd_only(gdb_ctx* ctx, int xcd);
+/* Bounds every device wait (default 10 s) and the lock-free close's quiet
+ * period (default 20 us); both in microseconds.  wait_us = 0 makes any wait
+ * that would block fail at once with GDB_E_TIMEOUT. */
+int gdb_ctx_set_timeouts(gdb_ctx* ctx, double wait_us, double quiet_us);
+int gdb_ctx_init(gdb_ctx* ctx);
+int gdb_ctx_init_skip(gdb_ctx* ctx, int skip);
+/* Returns P of the last launch (>= 0) or a negative error. */
+int gdb_number_of_participating_groups(gdb_ctx* ctx);
+/* Device-side sticky error of the last launch as a GDB_E_* code. */
+int gdb_ctx_device_error(gdb_ctx* ctx);
+void* gdb_ctx_stream(gdb_ctx* ctx);
+
+/* Launches `upper_bound` groups of `wgs` threads (with `lms` bytes of LDS)
+ * that only run the protocol; returns its device time in ns. */
+int gdb_time_protocol(gdb_ctx* ctx, int upper_bound, int wgs, int lms, double* ns);
+/* Protocol (prot=1) or skip mode (prot=0) followed by one barrier, with wgc
+ * groups of wgs threads and lms bytes of LDS.  Skip-mode launches beyond the
+ * residency bound would deadlock on any GPU and are refused (GDB_E_UNSAFE). */
+int gdb_occupancy_test(gdb_ctx* ctx, int wgc, int wgs, int lms, int prot, int* participating);
+/* Co-resident groups of the occupancy-test kernel for (wgs, lms). */
+int gdb_max_resident_groups(int wgs, int lms, int* out);
+/* Writes a human-readable device description. */
+int gdb_device_query(char* buf, size_t len);
+
+/* Barrier cost: a persistent kernel running `iters` barriers (or none, for
+ * iters = 0) on `groups` groups; returns device ms and the discovered P. */
+int gdb_barrier_bench(gdb_ctx* ctx, int wgs, int groups, int iters, double* ms, int* participating);
+/* Barrier invariants over `iters` barriers: per-round arrival counts equal
+ * P * k and plain stores made before a barrier are visible after it;
+ * returns the number of violations seen. */
+int gdb_barrier_check(gdb_ctx* ctx, int wgs, int groups, int iters, long long* violations,
+                      int* participating);
+/* Relaunch cost: `iters` launches of an empty kernel of `groups` groups.
+ * mode 0: back-to-back async launches, one final sync.
+ * mode 1: reference host loop, blocking 4-byte readback after every launch. */
+int gdb_relaunch_bench(gdb_ctx* ctx, int wgs, int groups, int iters, int mode, double* ms);
+
+/* ---- solvers (device pointers; CSR int32, row has n+1 entries) ------------- */
+/* Pannotia colouring (max rule): color[v] = round in which v was coloured. */
+int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const float* node_value,
+              int* color, const gdb_opts* opts, gdb_stats* stats);
+/* Pannotia MIS: s[v] = 2 for set members, 0 otherwise. */
+int gdb_mis(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const float* node_value,
+            int* s, const gdb_opts* opts, gdb_stats* stats);
+/* Pannotia pull Bellman-Ford over the transposed CSR (in-edges), source given;
+ * unreachable = 99999999. */
+int gdb_sssp_pull(gdb_ctx* ctx, int n, int m, const int* row_t, const int* col_t, const int* w_t,
+                  int source, int* dist, const gdb_opts* opts, gdb_stats* stats);
+/* Brandes betweenness over `nsrc` sources (FP32 sigma / delta / bc). */
+int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* row_t,
+           const int* col_t, const int* sources, int nsrc, float* bc, const gdb_opts* opts,
+           gdb_stats* stats);
+/* Lonestar worklist BFS: levels from `source`, unreachable = 1000000000. */
+int gdb_bfs_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col, int source,
+                     unsigned* dist, const gdb_opts* opts, gdb_stats* stats);
+/* Lonestar worklist SSSP (label-correcting, atomic min). */
+int gdb_sssp_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* w,
+                      int source, unsigned* dist, const gdb_opts* opts, gdb_stats* stats);
+/* dverifysolution: counts edges (u,v,w) with dist[u] + w < dist[v]; w = 1
+ * when `w` is NULL (BFS). */
+int gdb_verify_distances(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* w,
+                         const unsigned* dist, long long* nerr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDB_AMD_H */

@@ -1,0 +1,59 @@
+"""The C-ABI library loads and exports every symbol that include/*.h declares
+(no compute calls: this runs without a GPU)."""
+import re
+import subprocess
+from pathlib import Path
+
+import gpu_discovery_barrier_amd as gdb
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:int|void\s*\*?|long long)\s+(gdb_\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    syms = declared_symbols()
+    assert len(syms) > 20
+    lib = gdb.lib()
+    missing = [s for s in sorted(syms) if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_exports_are_c_linkage():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(gdb.api.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    assert declared_symbols() <= exported
+
+
+def test_generators_are_deterministic_and_symmetric():
+    a = gdb.gen_rmat(12, 16, seed=7, wlo=1, whi=100)
+    b = gdb.gen_rmat(12, 16, seed=7, wlo=1, whi=100)
+    assert (a.row == b.row).all() and (a.col == b.col).all() and (a.w == b.w).all()
+    t = a.transpose()
+    assert (t.row == a.row).all() and (t.col == a.col).all() and (t.w == a.w).all()
+    src = a.col.copy()
+    rows = __import__("numpy").repeat(__import__("numpy").arange(a.n), a.degrees())
+    assert not (rows == src).any()  # no self-loops
+    for i in range(0, a.n, 97):  # sorted, no duplicates
+        seg = a.col[a.row[i]:a.row[i + 1]]
+        assert (seg[1:] > seg[:-1]).all()
+    assert 1 <= a.w.min() and a.w.max() <= 100
+
+
+def test_grid_removes_exactly_five_percent():
+    g = gdb.gen_grid(64, 48, 0.05, seed=2)
+    undirected = 64 * 47 + 63 * 48
+    assert g.m == 2 * (undirected - round(0.05 * undirected))
+
+
+def test_gnp_average_degree():
+    g = gdb.gen_gnp(20000, 8.0, seed=4)
+    assert abs(g.m / g.n - 8.0) < 0.3
