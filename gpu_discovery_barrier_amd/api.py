@@ -288,8 +288,18 @@ def pannotia_values(n: int) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # Device graphs and the discovery context
 # ---------------------------------------------------------------------------
+def validate_csr(g: CSR) -> None:
+    """Rejects a CSR whose offsets or destinations would index out of range."""
+    row, col = np.asarray(g.row), np.asarray(g.col)
+    if row.size != g.n + 1 or row[0] != 0 or row[-1] != col.size or (np.diff(row) < 0).any():
+        raise ValueError("malformed CSR offsets")
+    if col.size and (col.min() < 0 or col.max() >= g.n):
+        raise ValueError("CSR destination out of range")
+
+
 class DeviceCSR:
     def __init__(self, g: CSR, weights: bool = True):
+        validate_csr(g)
         self.n, self.m = g.n, g.m
         self.row = DeviceBuffer.from_numpy(g.row.astype(np.int32))
         self.col = DeviceBuffer.from_numpy(g.col.astype(np.int32))

@@ -1,0 +1,409 @@
+// Pannotia betweenness centrality (Brandes) on the discovery protocol.
+//
+// Per source s (code/experiments/pannotia/graph_app/bc/kernel/bc_kernel.cl):
+// clean_1d_array resets sigma (the dependency delta), rho (path counts) and
+// the distance (lines 95-116); the forward BFS sets d and accumulates
+// rho[w] += rho[v] over edges into the next level (mega_bfs_kernel_func,
+// 295-371); the backward pass, level by level from the deepest, adds
+// rho[w]/rho[v] * (1 + delta[v]) into delta[w] for predecessors w read from
+// the transposed CSR and credits bc[v] += delta[v] for v != s
+// (mega_backtrack_kernel_func, 373-428).  No 1/2 factor.
+//
+// The reference's persistent kernel loops over every vertex as a source and
+// scales its reset stride by the workgroup size (line 454), so resets skip
+// vertices; the n^2 array `p` is never read.  Here the caller passes the
+// source list and every vertex is reset.
+//
+// Schedules:
+//   plain persistent - the mega kernel's structure: every level scans all n
+//                      vertices; rho and delta accumulate with float atomics.
+//   tuned persistent - level queues, one barrier per level: the phase that
+//                      expands level l (claiming level l+1 vertices with a
+//                      compare-and-swap on d) also pulls rho for the level-l
+//                      vertices from their in-edges; the backward pass pulls
+//                      delta from out-edges.  No float atomics, so results do
+//                      not depend on scheduling.
+//   relaunch         - bfs_kernel / backtrack_kernel per level with a host
+//                      loop (bc.cpp:322-419).
+#include "graph_common.hpp"
+#include "runtime.hpp"
+
+namespace {
+
+constexpr int kSerialLimit = 16;
+
+__device__ __forceinline__ void fatomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// Level of queue slot i (a corrupt entry reads as unreachable).
+__device__ __forceinline__ int level_of(const int* d, const int* q, int i, int n) {
+  const int v = q[i];
+  return (unsigned)v < (unsigned)n ? d[v] : -1;
+}
+
+template <int WGS>
+__global__ __launch_bounds__(WGS) void k_bc_plain(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
+                                                  const int* __restrict__ col,
+                                                  const int* __restrict__ row_t,
+                                                  const int* __restrict__ col_t, int* d, float* rho,
+                                                  float* delta, float* bc, const int* sources,
+                                                  int nsrc, int* stops, int n, int* out) {
+  DISCOVERY_PROTOCOL(ctx);
+  const int tid = p_get_global_id(ctx, &local_ctx);
+  const int stride = p_get_global_size(ctx, &local_ctx);
+  int nbar = 0, levels = 0;
+  for (int i = tid; i < n; i += stride) bc[i] = 0.f;  // clean_bc
+  for (int si = 0; si < nsrc; ++si) {
+    const int s = sources[si];
+    for (int i = tid; i < n; i += stride) {  // clean_1d_array
+      delta[i] = 0.f;
+      rho[i] = i == s ? 1.f : 0.f;
+      d[i] = i == s ? 0 : -1;
+    }
+    if (tid == 0) {
+      gdb::st_rlx(&stops[0], 0);
+      gdb::st_rlx(&stops[1], 0);
+    }
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    // Forward BFS with path counts (mega_bfs_kernel_func).
+    int ld = 0;
+    for (;;) {
+      bool cont = false;
+      for (int i = tid; i < n; i += stride) {
+        if (d[i] != ld) continue;
+        const float ri = rho[i];
+        for (int e = row[i]; e < row[i + 1]; ++e) {
+          const int w = col[e];
+          if (d[w] < 0) {
+            cont = true;
+            d[w] = ld + 1;
+          }
+          if (d[w] == ld + 1) fatomic_add(&rho[w], ri);
+        }
+      }
+      if (cont) gdb::st_rlx(&stops[ld % 3], 1);
+      if (tid == 0) gdb::st_rlx(&stops[(ld + 1) % 3], 0);
+      ++ld;
+      if (!discovery_barrier(ctx, &local_ctx)) return;
+      ++nbar;
+      if (gdb::ld_rlx(&stops[(ld - 1) % 3]) == 0) break;
+    }
+    levels += ld;
+    // Backward accumulation (mega_backtrack_kernel_func).
+    while (ld > 0) {
+      for (int i = tid; i < n; i += stride) {
+        if (d[i] != ld - 1) continue;
+        const float ri = rho[i], di = delta[i];
+        for (int e = row_t[i]; e < row_t[i + 1]; ++e) {
+          const int w = col_t[e];
+          if (d[w] == ld - 2) fatomic_add(&delta[w], rho[w] / ri * (1.f + di));
+        }
+        if (i != s) bc[i] = bc[i] + di;
+      }
+      --ld;
+      if (!discovery_barrier(ctx, &local_ctx)) return;
+      ++nbar;
+    }
+  }
+  if (tid == 0) {
+    out[0] = levels;
+    out[1] = nbar;
+  }
+}
+
+// Level-queue BC.  q holds the reached vertices of the current source level
+// by level; lvl_off[l] is the start of level l in q (lvl_off has n+2 slots).
+template <int WGS>
+__global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
+                                                  const int* __restrict__ col,
+                                                  const int* __restrict__ row_t,
+                                                  const int* __restrict__ col_t, int* d, float* sigma,
+                                                  float* delta, float* bc, const int* sources,
+                                                  int nsrc, int* q, int* tail, int n, int* out) {
+  DISCOVERY_PROTOCOL(ctx);
+  const int tid = p_get_global_id(ctx, &local_ctx);
+  const int stride = p_get_global_size(ctx, &local_ctx);
+  int nbar = 0, levels = 0;
+  for (int i = tid; i < n; i += stride) {
+    bc[i] = 0.f;
+    d[i] = -1;
+    sigma[i] = 0.f;
+    delta[i] = 0.f;
+  }
+  int prev_reached = 0;
+  for (int si = 0; si < nsrc; ++si) {
+    const int s = sources[si];
+    // Reset what the previous source touched (its reached set is q[0..)).
+    for (int i = tid; i < prev_reached; i += stride) {
+      const int v = q[i];
+      if ((unsigned)v >= (unsigned)n) continue;
+      d[v] = -1;
+      sigma[v] = 0.f;
+      delta[v] = 0.f;
+    }
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    if (tid == 0) {
+      d[s] = 0;
+      sigma[s] = 1.f;
+      q[0] = s;
+      gdb::st_rlx(tail, 1);
+    }
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    // Forward: level l occupies q[lb, le).  Expanding level l claims level
+    // l+1 vertices; the same phase pulls sigma for level l (l >= 1) from
+    // in-edges whose source sits on level l-1.
+    // Level boundaries live in registers: every group reads the same tail
+    // after each barrier.
+    int lb = 0, le = 1, l = 0;
+    for (;;) {
+      GDB_FOR_UNIFORM(i, valid, tid, stride, le - lb) {
+        int v = -1, b = 0, e = 0, bt = 0, et = 0;
+        if (valid) {
+          v = q[lb + i];
+          if ((unsigned)v >= (unsigned)n) valid = 0;
+        }
+        if (valid) {
+          b = row[v];
+          e = row[v + 1];
+          if (l > 0) {
+            bt = row_t[v];
+            et = row_t[v + 1];
+          }
+        }
+        if (l > 0) {
+          auto edge = [&](int k) -> float {
+            const int u = col_t[k];
+            return d[u] == l - 1 ? sigma[u] : 0.f;
+          };
+          auto red = [](float a, float c) { return a + c; };
+          const float sg =
+              gdb::wave_coop_reduce(valid != 0, bt, et, 0.f, edge, red, kSerialLimit);
+          if (valid) sigma[v] = sg;
+        }
+        gdb::wave_coop_foreach(
+            valid != 0, b, e,
+            [&](int k, bool ok) {
+              bool push = false;
+              int w = 0;
+              if (ok) {
+                w = col[k];
+                if (d[w] == -1) push = atomicCAS(&d[w], -1, l + 1) == -1;
+              }
+              gdb::wave_push(push, w, q, tail);
+            },
+            kSerialLimit);
+      }
+      if (!discovery_barrier(ctx, &local_ctx)) return;
+      ++nbar;
+      const int t = gdb::ld_rlx(tail);
+      if (t > n || t < le) {  // each vertex is claimed once: never more than n
+        gdb::set_error(ctx, GDB_ERR_QUEUE_OVERFLOW);
+        return;
+      }
+      if (t == le) break;  // level l had no successors: l is the deepest level
+      lb = le;
+      le = t;
+      ++l;
+    }
+    // q[lb, le) is the deepest level L = l; reached set is q[0, le).
+    const int reached = le;
+    levels += l + 1;
+    // Backward: delta[w] for w on level L-1 down to 0, pulled from out-edges
+    // into level +1.  q is ordered by level, so each level's range is found
+    // by binary search on d over q.
+    for (int ll = l - 1; ll >= 0; --ll) {
+      // [qb, qe) = positions of level ll in q.
+      int lo = 0, hi = reached;
+      while (lo < hi) {  // first position with d >= ll
+        const int mid = (lo + hi) >> 1;
+        if (level_of(d, q, mid, n) < ll) lo = mid + 1; else hi = mid;
+      }
+      const int qb = lo;
+      hi = reached;
+      while (lo < hi) {  // first position with d >= ll + 1
+        const int mid = (lo + hi) >> 1;
+        if (level_of(d, q, mid, n) < ll + 1) lo = mid + 1; else hi = mid;
+      }
+      const int qe = lo;
+      GDB_FOR_UNIFORM(i, valid, tid, stride, qe - qb) {
+        int w = -1, b = 0, e = 0;
+        if (valid) {
+          w = q[qb + i];
+          if ((unsigned)w >= (unsigned)n) valid = 0;
+        }
+        if (valid) {
+          b = row[w];
+          e = row[w + 1];
+        }
+        const float sw = valid ? sigma[w] : 0.f;
+        auto edge = [&](int k) -> float {
+          const int v = col[k];
+          return d[v] == ll + 1 ? (1.f + delta[v]) / sigma[v] : 0.f;
+        };
+        auto red = [](float a, float c) { return a + c; };
+        const float acc = gdb::wave_coop_reduce(valid != 0, b, e, 0.f, edge, red, kSerialLimit);
+        if (valid) {
+          const float dw = sw * acc;
+          delta[w] = dw;
+          if (w != s) bc[w] += dw;
+        }
+      }
+      if (!discovery_barrier(ctx, &local_ctx)) return;
+      ++nbar;
+    }
+    prev_reached = reached;
+  }
+  if (tid == 0) {
+    out[0] = levels;
+    out[1] = nbar;
+  }
+}
+
+// ---- relaunch: bfs_kernel (163-203) / backtrack_kernel (221-260) per level.
+__global__ void k_bc_clean(int* d, float* rho, float* delta, int s, int n, int* cont) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {  // clean_1d_array
+    delta[i] = 0.f;
+    rho[i] = i == s ? 1.f : 0.f;
+    d[i] = i == s ? 0 : -1;
+  }
+  if (i == 0) *cont = 0;
+}
+__global__ void k_bc_zero(float* bc, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) bc[i] = 0.f;
+}
+__global__ void k_bc_bfs(const int* __restrict__ row, const int* __restrict__ col, int* d, float* rho,
+                         int* cont, int n, int dist) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid < n && d[tid] == dist) {
+    const float r = rho[tid];
+    for (int e = row[tid]; e < row[tid + 1]; ++e) {
+      const int w = col[e];
+      if (d[w] < 0) {
+        *cont = 1;
+        d[w] = dist + 1;
+      }
+      if (d[w] == dist + 1) unsafeAtomicAdd(&rho[w], r);
+    }
+  }
+}
+__global__ void k_bc_back(const int* __restrict__ row_t, const int* __restrict__ col_t, const int* d,
+                          const float* rho, float* delta, int n, int dist, int s, float* bc) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid < n && d[tid] == dist - 1) {
+    const float r = rho[tid], di = delta[tid];
+    for (int e = row_t[tid]; e < row_t[tid + 1]; ++e) {
+      const int w = col_t[e];
+      if (d[w] == dist - 2) unsafeAtomicAdd(&delta[w], rho[w] / r * (1.f + di));
+    }
+    if (tid != s) bc[tid] = bc[tid] + di;
+  }
+}
+
+}  // namespace
+
+using namespace gdbrt;
+
+extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* row_t,
+                      const int* col_t, const int* sources, int nsrc, float* bc, const gdb_opts* opts,
+                      gdb_stats* stats) {
+  if (!ctx || n < 0 || m < 0 || nsrc < 0 || (n > 0 && (!row || !row_t || !bc)) ||
+      (m > 0 && (!col || !col_t)) || (nsrc > 0 && !sources))
+    return GDB_E_INVALID;
+  gdb_opts o{};
+  if (opts) o = *opts;
+  const int wgs = default_wgs(&o);
+  gdb_stats st{};
+  if (n == 0) {
+    if (stats) *stats = st;
+    return GDB_SUCCESS;
+  }
+  // Host copy of the sources for validation and the relaunch loop.
+  std::vector<int> hsrc((size_t)nsrc);
+  if (nsrc) GDB_HIP(hipMemcpy(hsrc.data(), sources, sizeof(int) * nsrc, hipMemcpyDeviceToHost));
+  for (int s : hsrc)
+    if (s < 0 || s >= n) return GDB_E_INVALID;
+  const size_t bytes = carve_bytes<int>(n) * 2 + carve_bytes<float>(n) * 2 + carve_bytes<int>(32);
+  int rc = ensure_workspace(ctx, bytes);
+  if (rc) return rc;
+  Carve cv(ctx->ws);
+  int* d = cv.take<int>(n);
+  int* q = cv.take<int>(n);
+  float* rho = cv.take<float>(n);
+  float* delta = cv.take<float>(n);
+  int* flags = cv.take<int>(32);
+  rc = fill_i32(ctx, flags, 0, 32);
+  if (rc) return rc;
+  rc = prepare_launch(ctx);
+  if (rc) return rc;
+  const int grid = (n + wgs - 1) / wgs;
+
+  if (o.schedule == GDB_SCHED_RELAUNCH) {
+    rc = timer_start(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bc_zero, dim3(grid), dim3(wgs), 0, ctx->stream, bc, n);
+    st.launches += 1;
+    for (int s : hsrc) {
+      hipLaunchKernelGGL(k_bc_clean, dim3(grid), dim3(wgs), 0, ctx->stream, d, rho, delta, s, n, flags);
+      st.launches += 1;
+      int dist = 0;
+      for (;;) {  // bc.cpp:363-390: launch a level, read `cont`
+        rc = fill_i32(ctx, flags, 0, 1);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_bc_bfs, dim3(grid), dim3(wgs), 0, ctx->stream, row, col, d, rho, flags, n,
+                           dist);
+        st.launches += 1;
+        GDB_HIP(hipMemcpyAsync(ctx->h_flag, flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        GDB_HIP(hipStreamSynchronize(ctx->stream));
+        ++dist;
+        if (ctx->h_flag[0] == 0) break;
+      }
+      st.rounds += dist;
+      for (; dist > 0; --dist) {  // bc.cpp:395-416
+        hipLaunchKernelGGL(k_bc_back, dim3(grid), dim3(wgs), 0, ctx->stream, row_t, col_t, d, rho, delta,
+                           n, dist, s, bc);
+        st.launches += 1;
+      }
+      GDB_HIP(hipGetLastError());
+    }
+    rc = timer_stop(ctx, &st.kernel_ms);
+    if (rc) return rc;
+    st.participating = grid;
+  } else {
+    const bool tuned = o.variant == GDB_VARIANT_TUNED;
+    int* out = flags + 8;
+    rc = gdb_dispatch_wgs(wgs, [&](auto W) -> int {
+      constexpr int WG = decltype(W)::value;
+      const void* kern = tuned ? (const void*)k_bc_tuned<WG> : (const void*)k_bc_plain<WG>;
+      int groups = 0;
+      int r2 = launch_groups(ctx, kern, &o, n, &groups);
+      if (r2) return r2;
+      r2 = timer_start(ctx);
+      if (r2) return r2;
+      if (tuned)
+        hipLaunchKernelGGL(k_bc_tuned<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col,
+                           row_t, col_t, d, rho, delta, bc, sources, nsrc, q, flags + 4, n, out);
+      else
+        hipLaunchKernelGGL(k_bc_plain<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col,
+                           row_t, col_t, d, rho, delta, bc, sources, nsrc, flags, n, out);
+      GDB_HIP(hipGetLastError());
+      return timer_stop(ctx, &st.kernel_ms);
+    });
+    if (rc) return rc;
+    st.launches = 1;
+    int hout[2] = {0, 0};
+    GDB_HIP(hipMemcpy(hout, out, sizeof(hout), hipMemcpyDeviceToHost));
+    st.rounds = hout[0];
+    st.barriers = hout[1];
+    rc = read_participating(ctx, &st.participating);
+    if (rc) return rc;
+  }
+  rc = check_device_error(ctx);
+  st.work_edges = (long long)m * nsrc;
+  st.work_items = (long long)n * nsrc;
+  if (stats) *stats = st;
+  return rc;
+}

@@ -24,11 +24,11 @@
 // Inter-workgroup barrier flavours.
 #define GDB_BAR_XF 0       // master/slave flags, discovery.cl:129-190
 #define GDB_BAR_COUNTER 1  // central sense-reversing arrival counter
-#define GDB_BAR_TREE 2     // per-XCD arrival, one L2 write-back per XCD, then XCD leaders
+#define GDB_BAR_TREE 2     // two-level counter: clusters of groups, then cluster leaders
 #define GDB_BAR_AUTO 3     // COUNTER for small P, TREE otherwise
-#define GDB_BAR_LOCAL 4    // all participants on one XCD: sync through its shared L2
 
-#define GDB_MAX_XCDS 16
+#define GDB_MAX_XCDS 16        // XCD ids read from the hardware are masked to this
+#define GDB_TREE_CLUSTERS 16   // clusters of the two-level counter
 #define GDB_AUTO_TREE_MIN_P 64
 
 // Error codes written to the context's error word and returned by the C ABI.
@@ -61,13 +61,10 @@ typedef struct {
   gdb_line_t timeout_ticks;   // bound on every wait (wall clock, 100 MHz ticks)
   gdb_line_t error;           // sticky GDB_ERR_* (0 = fine)
   gdb_line_t xcd_only;        // >= 0: only groups on this XCD participate
-  gdb_line_t epoch;           // launches completed (selects the membership buffer)
   // ---- counter / tree barrier state (self-resetting) -----------------------
   gdb_line_t bar_count;
   gdb_line_t bar_gen;
-  gdb_line_t xcd_count[GDB_MAX_XCDS];       // per-XCD arrivals
-  gdb_line_t xcd_gen[GDB_MAX_XCDS];         // per-XCD release generation
-  gdb_line_t xcd_members[2][GDB_MAX_XCDS];  // participants per XCD, by launch parity
+  gdb_line_t cluster_count[GDB_TREE_CLUSTERS];
   // ---- XF barrier flags ----------------------------------------------------
   gdb_line_t bar_flags[GDB_MAX_GROUPS];
 } gdb_kernel_ctx_t;

@@ -45,10 +45,6 @@ struct gdb_local_ctx {
   int bar_gen;    // generation of the global counter this group expects next
   int aborted;
   int xcd;        // XCD this group runs on (hardware register)
-  int epoch;      // launch parity selecting the membership buffer
-  int xmem;       // participants on this group's XCD (0 until the first tree barrier)
-  int nxcd;       // XCDs holding participants
-  int xgen;       // this XCD's release generation expected next
 };
 
 namespace gdb {
@@ -79,8 +75,6 @@ __device__ __forceinline__ int xchg(int* p, int v) {
 }
 __device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
-// L1 (per-CU vector cache) invalidate, no L2 maintenance.
-__device__ __forceinline__ void l1_inv() { asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory"); }
 // Waits until this wave's earlier stores have completed at L2.  A
 // workgroup-scope fence or __syncthreads() does not wait for them on gfx950,
 // so every wave runs this before a barrier's arrival is signalled.
@@ -142,19 +136,10 @@ __device__ inline void lock_release(gdb_kernel_ctx_t* ctx, int kind) {
     st_rel(&ctx->lock_now_serving.v, ld_rlx(&ctx->lock_now_serving.v) + 1);  // locks.cl:30-34
 }
 
-// End-of-launch bookkeeping done by the last launched group through the
-// closing phase: clear next launch's XCD membership buffer, advance the epoch.
-__device__ inline void advance_epoch(gdb_kernel_ctx_t* ctx, int epoch) {
-  const int nb = (epoch + 1) & 1;
-  for (int x = 0; x < GDB_MAX_XCDS; ++x) st_rlx(&ctx->xcd_members[nb][x].v, 0);
-  st_rlx(&ctx->epoch.v, epoch + 1);
-}
-
 // reset_kernel_context (discovery.cl:31-37).  Run by the last launched
 // group through the closing phase; the release store of the lock word is
 // also the unlock.
-__device__ inline void reset_locked_protocol(gdb_kernel_ctx_t* ctx, int epoch) {
-  advance_epoch(ctx, epoch);
+__device__ inline void reset_locked_protocol(gdb_kernel_ctx_t* ctx) {
   st_rlx(&ctx->prot_poll_open.v, 1);
   st_rlx(&ctx->prot_counter.v, 0);
   st_rlx(&ctx->kernel_counter.v, 0);
@@ -187,9 +172,8 @@ __device__ inline void protocol_locked(gdb_kernel_ctx_t* ctx, gdb_local_ctx* l, 
     st_rlx(&ctx->num_participating.v, ld_rlx(&ctx->prot_counter.v));
   }
   l->participating_group_size = ld_rlx(&ctx->prot_counter.v);
-  if (l->is_participating) fadd_rlx(&ctx->xcd_members[l->epoch & 1][l->xcd].v, 1);
   if (kc == total)
-    reset_locked_protocol(ctx, l->epoch);
+    reset_locked_protocol(ctx);
   else
     lock_release(ctx, kind);
 }
@@ -246,11 +230,9 @@ __device__ inline void protocol_lockfree(gdb_kernel_ctx_t* ctx, gdb_local_ctx* l
   l->participating_group_id = old & GDB_POLL_MASK;
   l->is_participating = part && P > 0 && l->participating_group_id < P;
   l->participating_group_size = P;
-  if (l->is_participating) fadd_rlx(&ctx->xcd_members[l->epoch & 1][l->xcd].v, 1);
   // The last launched group resets the words for the next launch.
   const int kc = fadd(&ctx->kernel_counter.v, 1);
   if (kc == total - 1) {
-    advance_epoch(ctx, l->epoch);
     st_rlx(&ctx->poll.v, 0);
     st_rlx(&ctx->closing_p.v, 0);
     st_rel(&ctx->kernel_counter.v, 0);
@@ -265,10 +247,6 @@ __device__ inline bool discovery_protocol(gdb_kernel_ctx_t* ctx, gdb_local_ctx* 
     l->aborted = 0;
     l->bar_gen = GDB_GEN_UNKNOWN;
     l->xcd = xcc_id();
-    l->xmem = 0;
-    l->nxcd = 0;
-    l->xgen = 0;
-    l->epoch = ld_acq(&ctx->epoch.v);
     int kind = ld_rlx(&ctx->bar_kind.v);
     const bool skip = ld_rlx(&ctx->skip.v) != 0;
     if (skip) {
@@ -283,14 +261,10 @@ __device__ inline bool discovery_protocol(gdb_kernel_ctx_t* ctx, gdb_local_ctx* 
       else
         protocol_locked(ctx, l, lk);
     }
-    // Resolve the barrier flavour.  Tree membership is only counted by the
-    // discovering protocols; one-XCD launches sync through that XCD's L2.
-    const bool one_xcd = !skip && ld_rlx(&ctx->lock_kind.v) == GDB_LOCK_FREE &&
-                         ld_rlx(&ctx->xcd_only.v) >= 0;
-    if (one_xcd) kind = GDB_BAR_LOCAL;
-    else if (kind == GDB_BAR_AUTO)
+    // Resolve the barrier flavour.
+    if (kind == GDB_BAR_AUTO)
       kind = l->participating_group_size >= GDB_AUTO_TREE_MIN_P ? GDB_BAR_TREE : GDB_BAR_COUNTER;
-    if ((kind == GDB_BAR_TREE && skip) || kind == GDB_BAR_LOCAL && !one_xcd) kind = GDB_BAR_COUNTER;
+    if (kind != GDB_BAR_XF && kind != GDB_BAR_TREE) kind = GDB_BAR_COUNTER;
     l->bar_kind = kind;
   }
   __syncthreads();
@@ -367,83 +341,41 @@ __device__ inline bool counter_barrier(gdb_kernel_ctx_t* ctx, gdb_local_ctx* l) 
   return l->aborted == 0;
 }
 
-// Tree barrier over XCDs.  Groups of one XCD share its L2: each drains its
-// stores to L2 and arrives on the XCD's counter; the XCD's last arriver
-// writes that L2 back once (agent release) and arrives on the global
-// counter; after the global generation moves it invalidates the L2 once
-// (agent acquire) and releases the XCD's other groups, which only need to
-// invalidate their L1.  One write-back and one invalidate per XCD instead
-// of one per group.
-__device__ inline void tree_setup(gdb_kernel_ctx_t* ctx, gdb_local_ctx* l) {
-  // First use in this launch: one flat barrier makes every participant's
-  // membership registration visible, then the counts are read.
-  if (threadIdx.x == 0) {
-    counter_wait(ctx, &ctx->bar_count.v, &ctx->bar_gen.v, l->participating_group_size, &l->bar_gen,
-                 &l->aborted);
-    int nx = 0;
-    for (int x = 0; x < GDB_MAX_XCDS; ++x) nx += ld_rlx(&ctx->xcd_members[l->epoch & 1][x].v) > 0;
-    l->nxcd = nx;
-    l->xmem = ld_rlx(&ctx->xcd_members[l->epoch & 1][l->xcd].v);
-    l->xgen = ld_rlx(&ctx->xcd_gen[l->xcd].v);
-  }
-}
-
+// Two-level counter: groups arrive on their cluster's counter (clusters of
+// consecutive participating ids, at most GDB_TREE_CLUSTERS of them, each counter
+// on its own line); a cluster's last arriver arrives on the global counter;
+// the last global arriver bumps the generation that every leader polls.
+// Every arrival is an agent-scope acq_rel read-modify-write, so each
+// group's writes reach the generation bump through a chain of
+// release/acquire pairs.  Contention on any one counter drops from P to
+// about P / GDB_TREE_CLUSTERS.
 __device__ inline bool tree_barrier(gdb_kernel_ctx_t* ctx, gdb_local_ctx* l) {
   drain_stores();
   __syncthreads();
-  if (l->xmem == 0) {
-    tree_setup(ctx, l);
-    __syncthreads();
-    return l->aborted == 0;  // the setup barrier is this barrier
-  }
   if (threadIdx.x == 0) {
-    const int x = l->xcd;
-    const int g = l->bar_gen, xg = l->xgen;
-    bool ok = true;
-    drain_stores();
-    const int old = fadd_rlx(&ctx->xcd_count[x].v, 1);
-    if (old == l->xmem - 1) {
-      st_rlx(&ctx->xcd_count[x].v, 0);
-      const int o2 = fadd(&ctx->bar_count.v, 1);  // release: this XCD's L2 written back
-      if (o2 == l->nxcd - 1) {
-        st_rlx(&ctx->bar_count.v, 0);
-        st_rel(&ctx->bar_gen.v, g + 1);
-      } else {
-        ok = wait_while(ctx, [&] { return ld_rlx(&ctx->bar_gen.v) == g; }, GDB_ERR_BARRIER_TIMEOUT);
-        fence_acq();
-      }
-      st_rlx(&ctx->xcd_gen[x].v, xg + 1);
-    } else {
-      ok = wait_while(ctx, [&] { return ld_rlx(&ctx->xcd_gen[x].v) == xg; }, GDB_ERR_BARRIER_TIMEOUT);
-      l1_inv();
-    }
-    l->bar_gen = g + 1;
-    l->xgen = xg + 1;
-    if (!ok) l->aborted = 1;
-  }
-  __syncthreads();
-  return l->aborted == 0;
-}
-
-// All participants on one XCD: stores drained to the shared L2, relaxed
-// counter, L1 invalidate.  No L2 maintenance is needed inside one L2.
-__device__ inline bool local_barrier(gdb_kernel_ctx_t* ctx, gdb_local_ctx* l) {
-  drain_stores();
-  __syncthreads();
-  if (threadIdx.x == 0) {
+    const int id = l->participating_group_id;
+    const int P = l->participating_group_size;
+    const int C = (P + GDB_TREE_CLUSTERS - 1) / GDB_TREE_CLUSTERS;
+    const int c = id / C;
+    const int members = min(C, P - c * C);
+    const int nclus = (P + C - 1) / C;
     int g = l->bar_gen;
     if (g == GDB_GEN_UNKNOWN) g = ld_acq(&ctx->bar_gen.v);
-    bool ok = true;
-    drain_stores();
-    const int old = fadd_rlx(&ctx->bar_count.v, 1);
-    if (old == l->participating_group_size - 1) {
-      st_rlx(&ctx->bar_count.v, 0);
-      drain_stores();
-      st_rlx(&ctx->bar_gen.v, g + 1);
-    } else {
-      ok = wait_while(ctx, [&] { return ld_rlx(&ctx->bar_gen.v) == g; }, GDB_ERR_BARRIER_TIMEOUT);
+    bool ok = true, waiter = true;
+    const int old = fadd(&ctx->cluster_count[c].v, 1);
+    if (old == members - 1) {
+      st_rlx(&ctx->cluster_count[c].v, 0);
+      const int o2 = fadd(&ctx->bar_count.v, 1);
+      if (o2 == nclus - 1) {
+        st_rlx(&ctx->bar_count.v, 0);
+        st_rel(&ctx->bar_gen.v, g + 1);
+        waiter = false;
+      }
     }
-    l1_inv();
+    if (waiter) {
+      ok = wait_while(ctx, [&] { return ld_rlx(&ctx->bar_gen.v) == g; }, GDB_ERR_BARRIER_TIMEOUT);
+      fence_acq();
+    }
     l->bar_gen = g + 1;
     if (!ok) l->aborted = 1;
   }
@@ -460,7 +392,6 @@ __device__ inline bool barrier(gdb_kernel_ctx_t* ctx, gdb_local_ctx* l) {
   switch (l->bar_kind) {
     case GDB_BAR_XF: return xf_barrier(ctx, l);
     case GDB_BAR_TREE: return tree_barrier(ctx, l);
-    case GDB_BAR_LOCAL: return local_barrier(ctx, l);
     default: return counter_barrier(ctx, l);
   }
 }

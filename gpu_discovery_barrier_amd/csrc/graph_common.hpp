@@ -47,6 +47,46 @@ __device__ __forceinline__ T wave_coop_reduce(bool active, int b, int e, T ident
   return acc;
 }
 
+// Thread / wave cooperative walk over the edge ranges [b, e) of the lanes of
+// one wave, calling fn(k, valid) in wave-uniform steps so that fn may use
+// wave collectives (ballot-based queue pushes).  Short ranges are walked by
+// their own lane (trip count = the longest short range in the wave); long
+// ranges by all 64 lanes, 64 consecutive edges per step.  Lanes without an
+// edge in a step get valid = false.
+template <typename F>
+__device__ __forceinline__ void wave_coop_foreach(bool active, int b, int e, F fn, int serial_limit) {
+  const bool heavy = active && (e - b) > serial_limit;
+  int len = (active && !heavy) ? (e - b) : 0;
+  int maxlen = len;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(maxlen, off, 64);
+    maxlen = o > maxlen ? o : maxlen;
+  }
+  for (int t = 0; t < maxlen; ++t) fn(b + t, t < len);
+  unsigned long long mask = __ballot(heavy);
+  const int lane = lane_id();
+  while (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    const int lb = __shfl(b, leader, 64);
+    const int le = __shfl(e, leader, 64);
+    for (int base = lb; base < le; base += 64) fn(base + lane, base + lane < le);
+    mask &= mask - 1;
+  }
+}
+
+// Appends `item` for every lane with push == true to queue[*tail ...]:
+// one atomic per wave.  Returns nothing; all 64 lanes must call it.
+__device__ __forceinline__ void wave_push(bool push, int item, int* queue, int* tail) {
+  const unsigned long long m = __ballot(push);
+  if (m == 0ull) return;
+  const int lane = lane_id();
+  int base = 0;
+  if (lane == __ffsll((long long)m) - 1) base = fadd_rlx(tail, __popcll(m));
+  base = __shfl(base, __ffsll((long long)m) - 1, 64);
+  if (push) queue[base + __popcll(m & ((1ull << lane) - 1ull))] = item;
+}
+
 // Wave-wide OR of a predicate.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 

@@ -35,6 +35,13 @@ __global__ void gdb_k_init_ctx(gdb_kernel_ctx_t* ctx, int skip, int lock_kind, i
   gdb::st_rlx(&lines[i].v, v);
 }
 
+// Fills `count` ints with `value` (used instead of hipMemsetAsync so that a
+// captured stream never records memset nodes).
+__global__ void gdb_k_fill_i32(int* p, int value, long long count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = value;
+}
+
 // Clears the sticky error word before a solve.
 __global__ void gdb_k_clear_error(gdb_kernel_ctx_t* ctx) { gdb::st_rlx(&ctx->error.v, GDB_OK); }
 
@@ -110,6 +117,14 @@ int ensure_workspace(gdb_ctx* ctx, size_t bytes) {
 
 int prepare_launch(gdb_ctx* ctx) {
   hipLaunchKernelGGL(gdb_k_clear_error, dim3(1), dim3(1), 0, ctx->stream, ctx->d);
+  GDB_HIP(hipGetLastError());
+  return GDB_SUCCESS;
+}
+
+int fill_i32(gdb_ctx* ctx, int* p, int value, long long count) {
+  if (count <= 0) return GDB_SUCCESS;
+  const long long blocks = (count + 255) / 256;
+  hipLaunchKernelGGL(gdb_k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, p, value, count);
   GDB_HIP(hipGetLastError());
   return GDB_SUCCESS;
 }
@@ -436,9 +451,10 @@ int gdb_barrier_check(gdb_ctx* ctx, int wgs, int groups, int iters, long long* v
   int* count = cv.take<int>(4);
   int* data = cv.take<int>((size_t)2 * stride);
   unsigned long long* bad = cv.take<unsigned long long>(1);
-  GDB_HIP(hipMemsetAsync(count, 0, 4 * sizeof(int), ctx->stream));
-  GDB_HIP(hipMemsetAsync(data, 0xff, (size_t)2 * stride * sizeof(int), ctx->stream));
-  GDB_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), ctx->stream));
+  rc = fill_i32(ctx, count, 0, 4);
+  if (!rc) rc = fill_i32(ctx, data, -1, (long long)2 * stride);
+  if (!rc) rc = fill_i32(ctx, (int*)bad, 0, 2);
+  if (rc) return rc;
   hipLaunchKernelGGL(gdb_k_barrier_check, dim3(groups), dim3(wgs), 0, ctx->stream, ctx->d, iters,
                      count, data, stride, bad);
   GDB_HIP(hipGetLastError());

@@ -58,6 +58,9 @@ struct Carve {
 template <typename T>
 inline size_t carve_bytes(size_t count) { return (count * sizeof(T) + 255) & ~(size_t)255; }
 
+// Fills count ints with value on the context stream (a kernel, not a memset).
+int fill_i32(gdb_ctx* ctx, int* p, int value, long long count);
+
 // Writes the per-launch protocol configuration and clears the sticky error.
 int prepare_launch(gdb_ctx* ctx);
 

@@ -194,7 +194,7 @@ int orc_sssp_pull_ref(int n, const int* row_t, const int* col_t, const int* w_t,
     if (stop == 0) break;
   }
   free(y);
-  return k;
+  return k < n ? k : n - 1; /* rounds executed */
 }
 
 typedef struct {

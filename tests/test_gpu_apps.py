@@ -1,0 +1,185 @@
+"""Parity of MIS, pull SSSP, BC and the worklist BFS / SSSP with the oracle,
+for every schedule, through the C ABI.
+
+Bars: MIS sets, SSSP distances and BFS levels bit-exact; BC (FP32) within
+1e-4 relative of the FP64 Brandes oracle, with an absolute floor of 1e-4 times
+the largest value for entries near zero."""
+import numpy as np
+import pytest
+
+import gpu_discovery_barrier_amd as gdb
+import oracle
+from conftest import csr_from_edges, grid_csr, load_fixtures, values_of
+
+pytestmark = pytest.mark.gpu
+FX = load_fixtures()
+
+SCHEDS = [
+    dict(schedule=gdb.SCHED_PERSISTENT, variant=gdb.VARIANT_PLAIN),
+    dict(schedule=gdb.SCHED_PERSISTENT, variant=gdb.VARIANT_TUNED),
+    dict(schedule=gdb.SCHED_RELAUNCH, variant=gdb.VARIANT_PLAIN),
+    dict(schedule=gdb.SCHED_RELAUNCH, variant=gdb.VARIANT_PLAIN, relaunch_check=5),
+]
+SID = lambda s: "-".join(map(str, s.values()))
+BC_RTOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gdb.Context()
+    yield c
+    c.close()
+
+
+def dev(row, col, w=None):
+    w = np.ones_like(col) if w is None else w
+    return gdb.DeviceCSR(gdb.CSR(row.size - 1, row, col, w))
+
+
+def bc_close(got, want):
+    want = np.asarray(want, np.float64)
+    floor = 1e-4 * max(1.0, float(np.abs(want).max()))
+    err = np.abs(got.astype(np.float64) - want)
+    ok = err <= np.maximum(BC_RTOL * np.abs(want), floor)
+    assert ok.all(), (np.flatnonzero(~ok)[:10], got[~ok][:10], want[~ok][:10])
+
+
+# ---------------------------------------------------------------- MIS
+@pytest.mark.parametrize("sched", SCHEDS, ids=SID)
+@pytest.mark.parametrize("fx", FX["mis"], ids=lambda f: f["name"])
+def test_mis_fixtures(ctx, fx, sched):
+    row, col, _ = csr_from_edges(fx["n"], fx["edges"])
+    got, _ = gdb.mis(ctx, dev(row, col), gdb.DeviceBuffer.from_numpy(values_of(fx)),
+                     gdb.Opts.make(wgs=64, **sched))
+    assert got.tolist() == fx["s"]
+
+
+@pytest.mark.parametrize("sched", SCHEDS, ids=SID)
+@pytest.mark.parametrize("vals", ["id", "perm"])
+def test_mis_rmat(ctx, sched, vals):
+    g = gdb.gen_rmat(15, 16, seed=3)
+    nv = gdb.pannotia_values(g.n) if vals == "id" else \
+        gdb.permutation(g.n, 3).astype(np.float32) / np.float32(g.n + 1)
+    want, rounds = oracle.mis_ref(g.row, g.col, nv)
+    got, st = gdb.mis(ctx, dev(g.row, g.col), gdb.DeviceBuffer.from_numpy(nv), gdb.Opts.make(**sched))
+    print(vals, SID(sched), st.as_dict())
+    assert (got == want).all()
+    assert st.rounds == rounds
+
+
+# ---------------------------------------------------------------- pull SSSP
+@pytest.mark.parametrize("sched", SCHEDS, ids=SID)
+@pytest.mark.parametrize("fx", FX["sssp_pull"], ids=lambda f: f["name"])
+def test_sssp_pull_fixtures(ctx, fx, sched):
+    row, col, w = csr_from_edges(fx["n"], fx["edges"], fx["w"])
+    got, _ = gdb.sssp_pull(ctx, dev(row, col, w), 0, gdb.Opts.make(wgs=64, **sched))
+    assert got.tolist() == fx["dist"]
+
+
+@pytest.mark.parametrize("sched", SCHEDS, ids=SID)
+def test_sssp_pull_rmat(ctx, sched):
+    g = gdb.gen_rmat(14, 16, seed=5, wlo=1, whi=100)
+    gt = g.transpose()
+    want, rounds = oracle.sssp_pull_ref(gt.row, gt.col, gt.w, 0)
+    got, st = gdb.sssp_pull(ctx, dev(gt.row, gt.col, gt.w), 0, gdb.Opts.make(**sched))
+    print(SID(sched), st.as_dict())
+    assert (got == want).all()
+    assert st.rounds == rounds
+
+
+@pytest.mark.parametrize("sched", SCHEDS[:2], ids=SID)
+def test_sssp_pull_round_cap_grid(ctx, sched):
+    # 30x3 grid: hop diameter 31 > n-1?  No: n = 90; use a path so the cap binds.
+    n = 40
+    row, col, w = csr_from_edges(n, [(i, i + 1) for i in range(n - 1)], [3] * (n - 1))
+    want, rounds = oracle.sssp_pull_ref(row, col, w, 0)
+    got, st = gdb.sssp_pull(ctx, dev(row, col, w), 0, gdb.Opts.make(wgs=64, **sched))
+    assert (got == want).all() and want[-1] == oracle.PANNOTIA_INF and st.rounds == rounds
+
+
+# ---------------------------------------------------------------- BC
+@pytest.mark.parametrize("sched", SCHEDS[:3], ids=SID)
+@pytest.mark.parametrize("fx", FX["bc"], ids=lambda f: f["name"])
+def test_bc_fixtures(ctx, fx, sched):
+    row, col, _ = csr_from_edges(fx["n"], fx["edges"])
+    g = dev(row, col)
+    got, _ = gdb.bc(ctx, g, g, np.array(fx["sources"], np.int32), gdb.Opts.make(wgs=64, **sched))
+    bc_close(got, fx["bc"])
+
+
+@pytest.mark.parametrize("sched", SCHEDS[:3], ids=SID)
+def test_bc_rmat(ctx, sched):
+    g = gdb.gen_rmat(13, 16, seed=3)
+    src = gdb.permutation(g.n, 3)[:16]
+    want = oracle.bc_brandes_tb(g.row, g.col, src)
+    d = dev(g.row, g.col)
+    got, st = gdb.bc(ctx, d, d, src, gdb.Opts.make(**sched))
+    print(SID(sched), st.as_dict())
+    bc_close(got, want)
+
+
+def test_bc_tuned_is_deterministic(ctx):
+    g = gdb.gen_rmat(12, 16, seed=8)
+    src = gdb.permutation(g.n, 8)[:8]
+    d = dev(g.row, g.col)
+    a, _ = gdb.bc(ctx, d, d, src, gdb.Opts.make(variant=gdb.VARIANT_TUNED))
+    b, _ = gdb.bc(ctx, d, d, src, gdb.Opts.make(variant=gdb.VARIANT_TUNED, wgs=1024, groups=7))
+    assert (a == b).all()
+
+
+# ---------------------------------------------------------------- worklist BFS / SSSP
+@pytest.mark.parametrize("sched", SCHEDS, ids=SID)
+@pytest.mark.parametrize("fx", FX["bfs"], ids=lambda f: f["name"])
+def test_bfs_fixtures(ctx, fx, sched):
+    if "rows" in fx:
+        row, col, _ = grid_csr(fx["rows"], fx["cols"])
+    else:
+        row, col, _ = csr_from_edges(fx["n"], fx["edges"])
+    got, _ = gdb.bfs_worklist(ctx, dev(row, col), 0, gdb.Opts.make(wgs=64, **sched))
+    assert got.tolist() == fx["dist"]
+
+
+@pytest.mark.parametrize("sched", SCHEDS, ids=SID)
+@pytest.mark.parametrize("fx", FX["sssp_worklist"], ids=lambda f: f["name"])
+def test_sssp_worklist_fixtures(ctx, fx, sched):
+    row, col, w = csr_from_edges(fx["n"], fx["edges"], fx["w"])
+    got, _ = gdb.sssp_worklist(ctx, dev(row, col, w), 0, gdb.Opts.make(wgs=64, **sched))
+    assert got.tolist() == fx["dist"]
+
+
+@pytest.mark.parametrize("sched", SCHEDS, ids=SID)
+def test_bfs_sssp_grid(ctx, sched):
+    g = gdb.gen_grid(256, 256, 0.05, seed=2, wlo=1, whi=255)
+    d = dev(g.row, g.col, g.w)
+    want_b, iters = oracle.bfs_worklist_ref(g.row, g.col, 0)
+    got_b, st = gdb.bfs_worklist(ctx, d, 0, gdb.Opts.make(**sched))
+    print("bfs", SID(sched), st.as_dict())
+    assert (got_b == want_b).all()
+    assert st.rounds == iters
+    want_s, _ = oracle.sssp_worklist_ref(g.row, g.col, g.w, 0)
+    got_s, st = gdb.sssp_worklist(ctx, d, 0, gdb.Opts.make(**sched))
+    print("sssp", SID(sched), st.as_dict())
+    assert (got_s == want_s).all()
+
+
+@pytest.mark.parametrize("sched", SCHEDS[:2], ids=SID)
+def test_bfs_sssp_rmat(ctx, sched):
+    g = gdb.gen_rmat(15, 16, seed=6, wlo=1, whi=255)
+    d = dev(g.row, g.col, g.w)
+    got_b, _ = gdb.bfs_worklist(ctx, d, 0, gdb.Opts.make(**sched))
+    assert (got_b == oracle.bfs_tb(g.row, g.col, 0)).all()
+    got_s, _ = gdb.sssp_worklist(ctx, d, 0, gdb.Opts.make(**sched))
+    assert (got_s.astype(np.int64) == oracle.dijkstra_tb(g.row, g.col, g.w, 0)).all()
+
+
+def test_verify_distances_kernel(ctx):
+    g = gdb.gen_grid(64, 64, 0.05, seed=2, wlo=1, whi=255)
+    d = dev(g.row, g.col, g.w)
+    want, _ = oracle.sssp_worklist_ref(g.row, g.col, g.w, 0)
+    good = gdb.DeviceBuffer.from_numpy(want)
+    assert gdb.verify_distances(ctx, d, good, weighted=True) == 0
+    bad = want.copy()
+    bad[100] += 1000
+    n_bad = oracle.verify_distances_ref(g.row, g.col, g.w, bad)
+    assert n_bad > 0
+    assert gdb.verify_distances(ctx, d, gdb.DeviceBuffer.from_numpy(bad), weighted=True) == n_bad
