@@ -183,8 +183,8 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
           if (valid) sigma[v] = sg;
         }
         gdb::wave_coop_foreach(
-            valid != 0, b, e,
-            [&](int k, bool ok) {
+            valid != 0, b, e, 0u,
+            [&](int k, bool ok, unsigned) {
               bool push = false;
               int w = 0;
               if (ok) {

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(WGS) void k_wl_tuned(gdb_kernel_ctx_t* ctx, const i
   const int tid = p_get_global_id(ctx, &local_ctx);
   const int threads = p_get_global_size(ctx, &local_ctx);
   int iteration = 1, nbar = 0;
-  long long items = 0;
+  long long items = 0, edges = 0;
   int nin = gdb::ld_rlx(&tails[1]);
   while (nin > 0) {
     if (nin > n) {
@@ -151,12 +151,14 @@ __global__ __launch_bounds__(WGS) void k_wl_tuned(gdb_kernel_ctx_t* ctx, const i
         b = row[v];
         e = row[v + 1];
         if (WEIGHTED) dv = dist[v];
+        edges += e - b;
       }
       gdb::wave_coop_foreach(
-          valid != 0, b, e,
-          [&](int k, bool ok) {
+          valid != 0, b, e, dv,
+          [&](int k, bool ok, unsigned src_dist) {
             int dst = 0;
-            const bool push = ok && relax_edge<WEIGHTED>(col, w, dist, stamp, k, dv, iteration, n, &dst);
+            const bool push =
+                ok && relax_edge<WEIGHTED>(col, w, dist, stamp, k, src_dist, iteration, n, &dst);
             gdb::wave_push(push, dst, outq, out_tail);
           },
           kSerialLimit);
@@ -166,6 +168,9 @@ __global__ __launch_bounds__(WGS) void k_wl_tuned(gdb_kernel_ctx_t* ctx, const i
     nin = gdb::ld_rlx(out_tail);
     ++iteration;
   }
+  // Edges examined, summed over threads (one atomic per wave).
+  for (int off = 32; off > 0; off >>= 1) edges += __shfl_xor(edges, off, 64);
+  if (gdb::lane_id() == 0 && edges) atomicAdd((unsigned long long*)&work[1], (unsigned long long)edges);
   if (tid == 0) {
     out[0] = iteration - 1;
     out[1] = nbar;
@@ -209,10 +214,11 @@ __global__ void k_wl_step(const int* __restrict__ row, const int* __restrict__ c
       if (WEIGHTED) dv = dist[v];
     }
     gdb::wave_coop_foreach(
-        valid != 0, b, e,
-        [&](int k, bool ok) {
+        valid != 0, b, e, dv,
+        [&](int k, bool ok, unsigned src_dist) {
           int dst = 0;
-          const bool push = ok && relax_edge<WEIGHTED>(col, w, dist, stamp, k, dv, iteration, n, &dst);
+          const bool push =
+              ok && relax_edge<WEIGHTED>(col, w, dist, stamp, k, src_dist, iteration, n, &dst);
           gdb::wave_push(push, dst, outq, out_tail);
         },
         kSerialLimit);
@@ -275,6 +281,8 @@ int run_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col, con
   long long* work = cv.take<long long>(2);
   int* tails = flags + 4;
   int* out = flags + 8;
+  rc = fill_i32(ctx, (int*)work, 0, 4);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_wl_init, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, dist, stamp, q0, q1,
                      flags, n, source);
   GDB_HIP(hipGetLastError());
@@ -333,7 +341,12 @@ int run_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col, con
     GDB_HIP(hipMemcpy(hout, out, sizeof(hout), hipMemcpyDeviceToHost));
     st.rounds = hout[0];
     st.barriers = hout[1];
-    if (tuned) GDB_HIP(hipMemcpy(&st.work_items, work, sizeof(long long), hipMemcpyDeviceToHost));
+    if (tuned) {
+      long long hw[2] = {0, 0};
+      GDB_HIP(hipMemcpy(hw, work, sizeof(hw), hipMemcpyDeviceToHost));
+      st.work_items = hw[0];
+      st.work_edges = hw[1];
+    }
     rc = read_participating(ctx, &st.participating);
     if (rc) return rc;
   }

@@ -48,13 +48,16 @@ __device__ __forceinline__ T wave_coop_reduce(bool active, int b, int e, T ident
 }
 
 // Thread / wave cooperative walk over the edge ranges [b, e) of the lanes of
-// one wave, calling fn(k, valid) in wave-uniform steps so that fn may use
-// wave collectives (ballot-based queue pushes).  Short ranges are walked by
-// their own lane (trip count = the longest short range in the wave); long
-// ranges by all 64 lanes, 64 consecutive edges per step.  Lanes without an
-// edge in a step get valid = false.
+// one wave, calling fn(k, valid, payload) in wave-uniform steps so that fn
+// may use wave collectives (ballot-based queue pushes).  `payload` is the
+// value the range's owner lane passed in (for example its vertex's distance):
+// a lane walking another lane's range sees the owner's payload.  Short ranges
+// are walked by their own lane (trip count = the longest short range in the
+// wave); long ranges by all 64 lanes, 64 consecutive edges per step.  Lanes
+// without an edge in a step get valid = false.
 template <typename F>
-__device__ __forceinline__ void wave_coop_foreach(bool active, int b, int e, F fn, int serial_limit) {
+__device__ __forceinline__ void wave_coop_foreach(bool active, int b, int e, unsigned payload, F fn,
+                                                  int serial_limit) {
   const bool heavy = active && (e - b) > serial_limit;
   int len = (active && !heavy) ? (e - b) : 0;
   int maxlen = len;
@@ -63,14 +66,15 @@ __device__ __forceinline__ void wave_coop_foreach(bool active, int b, int e, F f
     const int o = __shfl_xor(maxlen, off, 64);
     maxlen = o > maxlen ? o : maxlen;
   }
-  for (int t = 0; t < maxlen; ++t) fn(b + t, t < len);
+  for (int t = 0; t < maxlen; ++t) fn(b + t, t < len, payload);
   unsigned long long mask = __ballot(heavy);
   const int lane = lane_id();
   while (mask) {
     const int leader = __ffsll((long long)mask) - 1;
     const int lb = __shfl(b, leader, 64);
     const int le = __shfl(e, leader, 64);
-    for (int base = lb; base < le; base += 64) fn(base + lane, base + lane < le);
+    const unsigned lp = (unsigned)__shfl((int)payload, leader, 64);
+    for (int base = lb; base < le; base += 64) fn(base + lane, base + lane < le, lp);
     mask &= mask - 1;
   }
 }

@@ -57,6 +57,24 @@ int gdb_gen_permutation(int n, uint64_t seed, int* out);
 int gdb_gen_uniform_ints(long long count, int lo, int hi, uint64_t seed, int* out);
 /* In-edge CSR: row u of the output lists the sources of edges into u. */
 int gdb_csr_transpose(const gdb_host_csr* in, gdb_host_csr* out);
+/* GDB_GRAPH_OK if offsets are monotone from 0 to m and every destination is
+ * in [0, n). */
+int gdb_csr_validate(const gdb_host_csr* g);
+
+/* ---- files (edge multiset and in-row order follow the reference parser;
+ * row offsets are always a correct CSR, also for vertices without edges) -- */
+/* METIS .graph: "n m" then one 1-based neighbour list per vertex.  With
+ * directed = 0 the declared m counts undirected edges. */
+int gdb_load_metis(const char* path, int directed, gdb_host_csr* out);
+/* DIMACS "p sp n m" / "a u v w" (1-based).  directed = 0 adds v->u for every
+ * arc; transpose = 1 stores in-edges the way parseCOO_transpose does. */
+int gdb_load_dimacs(const char* path, int directed, int transpose, gdb_host_csr* out);
+/* Galois binary .gr version 1 (u64 header, u64 end offsets, u32 targets,
+ * 32-bit edge data). */
+int gdb_load_galois_gr(const char* path, gdb_host_csr* out);
+int gdb_write_galois_gr(const char* path, const gdb_host_csr* g);
+/* Lonestar .edges text: "n m" then m lines "src dst weight" (0-based). */
+int gdb_load_edges(const char* path, gdb_host_csr* out);
 
 #ifdef __cplusplus
 }
