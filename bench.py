@@ -100,43 +100,62 @@ def main() -> int:
         ds, ss = gdb.sssp_worklist(ctx, dg, 0, o, out=dist)
         return db, sb, ds, ss
 
-    # Launch-shape autotuning of the tuned engine (untimed).
+    out_b = gdb.DeviceBuffer.empty(g.n, np.uint32)
+    out_s = gdb.DeviceBuffer.empty(g.n, np.uint32)
+
+    def solve_batched(wgs, groups, delta):
+        # One launch, two teams: BFS and near/far SSSP side by side.
+        tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0),
+                 gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta)]
+        rounds, st = gdb.batch_solve(ctx, tasks, max(1, groups // 2),
+                                     gdb.Opts.make(wgs=wgs, groups=groups))
+        return rounds, st
+
+    # Launch-shape / band-width autotuning of the batched engine (untimed).
     if args.no_tune:
-        wgs, groups = 256, 0
+        wgs, groups, delta = 256, 64, 1024
     else:
-        def run(w, gr):
-            _, sb, _, ss = solve(gdb.VARIANT_TUNED, w, gr)
-            return sb.kernel_ms + ss.kernel_ms
-        best = autotune.tune(f"wl_bfs_sssp:grid{GRID_SIDE}:{g.m}", run, repeats=1, use_cache=False)
-        wgs, groups = best["wgs"], best["groups"]
-        log(f"autotune: wgs={wgs} groups={groups} ({best['ms']:.2f} ms); "
-            f"{len(best['trials'])} shapes tried")
+        trials = []
+        for w in (256, 512, 1024):
+            for gr in (16, 32, 64, 128, 256):
+                gr = min(gr, gdb.max_resident_groups(w, 0))
+                for dl in (512, 2048):
+                    _, st = solve_batched(w, gr, dl)
+                    trials.append((st.kernel_ms, w, gr, dl))
+        trials.sort()
+        _, wgs, groups, delta = trials[0]
+        log(f"autotune: wgs={wgs} groups={groups} delta={delta} ({trials[0][0]:.2f} ms); "
+            f"{len(trials)} shapes tried")
 
     for _ in range(args.warmup):
-        solve(gdb.VARIANT_TUNED, wgs, groups)
-    bfs_ms, sssp_ms, walls = [], [], []
+        solve_batched(wgs, groups, delta)
+    step_ms, walls = [], []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        db, sb, ds, ss = solve(gdb.VARIANT_TUNED, wgs, groups)
+        rounds, bst = solve_batched(wgs, groups, delta)
         walls.append(time.perf_counter() - t0)
-        bfs_ms.append(sb.kernel_ms)
-        sssp_ms.append(ss.kernel_ms)
+        step_ms.append(bst.kernel_ms)
+    db, ds = out_b.to_numpy(), out_s.to_numpy()
 
     inf = gdb.LONESTAR_INF
     mr = reached_edges(g, db, inf)
-    t_step_ms = float(np.median(np.array(bfs_ms) + np.array(sssp_ms)))
+    t_step_ms = float(np.median(step_ms))
     gteps = 2 * mr / (t_step_ms / 1e3) / 1e9
+
+    # Single-task tuned kernels (one solve per launch) at their own best shape.
+    tb, tsb, tds, tss = solve(gdb.VARIANT_TUNED, 512, 128)
+    tuned_ms = tsb.kernel_ms + tss.kernel_ms
 
     # Plain engine (reference kernel structure, default launch shape).
     pdb, psb, pds, pss = solve(gdb.VARIANT_PLAIN, 256, 0)
     plain_ms = psb.kernel_ms + pss.kernel_ms
     plain_gteps = 2 * mr / (plain_ms / 1e3) / 1e9
-    bit_exact = bool((pdb == db).all() and (pds == ds).all())
+    bit_exact = bool((pdb == db).all() and (pds == ds).all() and (tb == db).all() and (tds == ds).all())
     nerr = gdb.verify_distances(ctx, dg, gdb.DeviceBuffer.from_numpy(ds), weighted=True)
     nerr += gdb.verify_distances(ctx, dg, gdb.DeviceBuffer.from_numpy(db), weighted=False)
 
-    # Barrier vs relaunch at the tuned launch shape.
-    P = ss.participating
+    # Barrier vs relaunch at the team launch shape (one team = groups / 2).
+    P = max(1, groups // 2)
     K = 4000
     t0_ms, _ = ctx.barrier_bench(wgs, P, 0)
     tk_ms, pb = ctx.barrier_bench(wgs, P, K)
@@ -144,12 +163,12 @@ def main() -> int:
     us_relaunch = ctx.relaunch_bench(wgs, P, K, 0) * 1e3 / K
     us_relaunch_sync = ctx.relaunch_bench(wgs, P, 500, 1) * 1e3 / 500
 
-    # Roofline of the dominant kernel (the SSSP worklist kernel): algorithmic
-    # bytes = 16 B per processed edge (dst, weight, dist[src], dist[dst]) +
-    # 24 B per processed worklist item.
-    sssp_bytes = 16 * ss.work_edges + 24 * ss.work_items
-    sssp_kernel_ms = float(np.median(sssp_ms))
-    achieved = sssp_bytes / (sssp_kernel_ms / 1e3) / 1e9
+    # Roofline of the dominant kernel (the batched kernel): algorithmic bytes
+    # = BFS 8 B per processed edge + 24 B per processed vertex, SSSP 16 B per
+    # processed edge + 24 B per processed worklist item.
+    n_items, n_edges = bst.work_items, bst.work_edges
+    alg_bytes = 16 * n_edges + 24 * n_items
+    achieved = alg_bytes / (t_step_ms / 1e3) / 1e9
 
     out = {
         "metric": "GTEPS on 1 MI355X (batched+learned vs plain, bit-exact); us per XF barrier",
@@ -168,10 +187,12 @@ def main() -> int:
             "workload": f"Lonestar worklist BFS + SSSP, {GRID_SIDE}x{GRID_SIDE} 4-neighbour grid, "
                         "5% edges removed (seed 2), weights U[1,255], source 0",
             "n": g.n, "m": g.m, "m_reached": mr,
-            "engine": "tuned persistent", "wgs": wgs, "groups": groups, "participating": P,
-            "bfs_levels": sb.rounds, "sssp_iterations": ss.rounds,
+            "engine": "batched (BFS and near/far SSSP teams in one persistent launch)",
+            "wgs": wgs, "groups": groups, "team_groups": max(1, groups // 2), "delta": delta,
+            "participating": bst.participating, "bfs_levels": rounds[0], "sssp_iterations": rounds[1],
         },
-        "kernel_ms": {"bfs": round(float(np.median(bfs_ms)), 3), "sssp": round(sssp_kernel_ms, 3)},
+        "kernel_ms": round(t_step_ms, 3),
+        "tuned_single": {"gteps": round(2 * mr / (tuned_ms / 1e3) / 1e9, 4), "ms": round(tuned_ms, 3)},
         "plain": {"gteps": round(plain_gteps, 4), "ms": round(plain_ms, 3),
                   "participating": pss.participating, "barriers": psb.barriers + pss.barriers},
         "speedup_vs_plain": round(gteps / plain_gteps, 3),
@@ -186,9 +207,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": traffic_from_profiles("sssp"),
-            "kernel": "k_wl_tuned<WGS,true> (worklist SSSP)",
-            "algorithmic_bytes": int(sssp_bytes),
+            "traffic": traffic_from_profiles("batch"),
+            "kernel": "k_batch<WGS> (BFS + SSSP teams)",
+            "algorithmic_bytes": int(alg_bytes),
         },
     }
     if not args.no_cpu:

@@ -71,6 +71,26 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+TASK_BFS, TASK_SSSP, TASK_COLOR = 0, 1, 2
+
+
+class Task(C.Structure):
+    """gdb_task (include/gdb_amd.h): one (graph, algorithm) task of a batch."""
+    _fields_ = [
+        ("algo", C.c_int),
+        ("n", C.c_int),
+        ("m", C.c_int),
+        ("row", C.c_void_p),
+        ("col", C.c_void_p),
+        ("w", C.c_void_p),
+        ("source", C.c_int),
+        ("node_value", C.c_void_p),
+        ("out", C.c_void_p),
+        ("delta", C.c_int),
+        ("rounds", C.c_int),
+    ]
+
+
 class HostCSRStruct(C.Structure):
     _fields_ = [
         ("n", C.c_int),
@@ -129,6 +149,13 @@ def _declare(lib) -> None:
         "gdb_bfs_worklist": (I, [P, I, I, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_sssp_worklist": (I, [P, I, I, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_verify_distances": (I, [P, I, I, P, P, P, P, C.POINTER(C.c_longlong)]),
+        "gdb_batch_solve": (I, [P, C.POINTER(Task), I, I, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_load_metis": (I, [C.c_char_p, I, C.POINTER(HostCSRStruct)]),
+        "gdb_load_dimacs": (I, [C.c_char_p, I, I, C.POINTER(HostCSRStruct)]),
+        "gdb_load_galois_gr": (I, [C.c_char_p, C.POINTER(HostCSRStruct)]),
+        "gdb_write_galois_gr": (I, [C.c_char_p, C.POINTER(HostCSRStruct)]),
+        "gdb_load_edges": (I, [C.c_char_p, C.POINTER(HostCSRStruct)]),
+        "gdb_csr_validate": (I, [C.POINTER(HostCSRStruct)]),
     }
     for name, (res, args) in {**sig, **optional}.items():
         if name in optional and not hasattr(lib, name):
@@ -466,3 +493,60 @@ def verify_distances(ctx: Context, g: DeviceCSR, dist: DeviceBuffer, weighted: b
                                      g.w.ptr if weighted else None, dist.ptr, C.byref(nerr)),
           "gdb_verify_distances")
     return nerr.value
+
+
+# ---------------------------------------------------------------------------
+# Batched engine
+# ---------------------------------------------------------------------------
+def make_task(algo: int, g: "DeviceCSR", out: DeviceBuffer, source: int = 0,
+              node_value: DeviceBuffer | None = None, delta: int = 0) -> Task:
+    t = Task()
+    t.algo, t.n, t.m = algo, g.n, g.m
+    t.row, t.col = g.row.ptr, g.col.ptr
+    t.w = g.w.ptr if g.w is not None else None
+    t.source = source
+    t.node_value = node_value.ptr if node_value is not None else None
+    t.out = out.ptr
+    t.delta = delta
+    t.rounds = 0
+    return t
+
+
+def batch_solve(ctx: Context, tasks: list, team_groups: int = 1, opts=None):
+    """Runs the tasks in one persistent launch; returns (rounds per task, Stats)."""
+    arr = (Task * len(tasks))(*tasks)
+    st = Stats()
+    o = _opts(opts)
+    check(lib().gdb_batch_solve(ctx.h, arr, len(tasks), team_groups, C.byref(o), C.byref(st)),
+          "gdb_batch_solve")
+    return [arr[i].rounds for i in range(len(tasks))], st
+
+
+# ---------------------------------------------------------------------------
+# Graph files
+# ---------------------------------------------------------------------------
+def _load(fn, what, *args) -> CSR:
+    out = HostCSRStruct()
+    check(fn(*args, C.byref(out)), what)
+    return _from_struct(out)
+
+
+def load_metis(path: str, directed: bool = False) -> CSR:
+    return _load(lib().gdb_load_metis, "gdb_load_metis", str(path).encode(), int(directed))
+
+
+def load_dimacs(path: str, directed: bool = True, transpose: bool = False) -> CSR:
+    return _load(lib().gdb_load_dimacs, "gdb_load_dimacs", str(path).encode(), int(directed), int(transpose))
+
+
+def load_galois_gr(path: str) -> CSR:
+    return _load(lib().gdb_load_galois_gr, "gdb_load_galois_gr", str(path).encode())
+
+
+def load_edges(path: str) -> CSR:
+    return _load(lib().gdb_load_edges, "gdb_load_edges", str(path).encode())
+
+
+def write_galois_gr(path: str, g: CSR) -> None:
+    s = _to_struct(g)
+    check(lib().gdb_write_galois_gr(str(path).encode(), C.byref(s)), "gdb_write_galois_gr")

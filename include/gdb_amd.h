@@ -173,6 +173,33 @@ int gdb_sssp_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col
 int gdb_verify_distances(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* w,
                          const unsigned* dist, long long* nerr);
 
+/* ---- batched engine -------------------------------------------------------
+ * One persistent launch solves many (graph, algorithm) tasks: discovered
+ * groups form teams of `team_groups` groups; each team takes tasks from a
+ * shared counter and synchronises on its own sub-barrier (a team of one group
+ * uses workgroup barriers only).  Outputs equal the single-task solvers'. */
+#define GDB_TASK_BFS 0    /* Lonestar worklist BFS levels (u32, 1e9 unreachable) */
+#define GDB_TASK_SSSP 1   /* worklist SSSP distances (u32); near-far when delta > 0 */
+#define GDB_TASK_COLOR 2  /* Pannotia max-rule colouring (i32) */
+
+typedef struct {
+  int algo;                 /* GDB_TASK_* */
+  int n, m;
+  const int* row;           /* device CSR */
+  const int* col;
+  const int* w;             /* SSSP weights */
+  int source;               /* BFS / SSSP */
+  const float* node_value;  /* colour priorities */
+  void* out;                /* device output: u32 distances or i32 colours */
+  int delta;                /* SSSP near-far band width (0 = plain worklist) */
+  int rounds;               /* out: levels / iterations / colour rounds */
+} gdb_task;
+
+/* team_groups: groups per team (0 = 1).  opts->wgs and opts->groups pick the
+ * launch shape.  stats->rounds is the sum of task rounds. */
+int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups, const gdb_opts* opts,
+                    gdb_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

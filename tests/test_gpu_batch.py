@@ -1,0 +1,98 @@
+"""Batched engine: many tasks in one persistent launch, bit-exact with the
+oracle (BFS levels, SSSP distances, colours), for teams of one group
+(workgroup barriers only) and multi-group teams (team sub-barriers), with
+and without the near/far SSSP split."""
+import numpy as np
+import pytest
+
+import gpu_discovery_barrier_amd as gdb
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gdb.Context()
+    yield c
+    c.close()
+
+
+def small_graphs(count, seed):
+    rng = np.random.default_rng(seed)
+    gs = []
+    for i in range(count):
+        if i % 2 == 0:
+            n = int(np.exp(rng.uniform(np.log(64), np.log(4096))))
+            gs.append(gdb.gen_gnp(n, 8.0, seed=seed * 1000 + i, wlo=1, whi=100))
+        else:
+            s = int(rng.integers(8, 48))
+            gs.append(gdb.gen_grid(s, s, 0.0, seed=seed * 1000 + i, wlo=1, whi=100))
+    return gs
+
+
+def run_batch(ctx, gs, team_groups, delta, wgs=256, groups=0):
+    dgs, outs, vals, tasks, kinds = [], [], [], [], []
+    for gi, g in enumerate(gs):
+        dg = gdb.DeviceCSR(g)
+        dgs.append(dg)
+        nv = gdb.DeviceBuffer.from_numpy(gdb.permutation(g.n, gi + 1).astype(np.float32) / np.float32(g.n + 1))
+        vals.append(nv)
+        for algo in (gdb.TASK_BFS, gdb.TASK_SSSP, gdb.TASK_COLOR):
+            out = gdb.DeviceBuffer.empty(g.n, np.int32 if algo == gdb.TASK_COLOR else np.uint32)
+            outs.append(out)
+            tasks.append(gdb.make_task(algo, dg, out, source=0, node_value=nv,
+                                       delta=delta if algo == gdb.TASK_SSSP else 0))
+            kinds.append((gi, algo))
+    rounds, st = gdb.batch_solve(ctx, tasks, team_groups, gdb.Opts.make(wgs=wgs, groups=groups))
+    return kinds, outs, vals, rounds, st
+
+
+def check(gs, kinds, outs, vals, rounds):
+    for (gi, algo), out, r in zip(kinds, outs, rounds):
+        g = gs[gi]
+        got = out.to_numpy()
+        if algo == gdb.TASK_BFS:
+            want, iters = oracle.bfs_worklist_ref(g.row, g.col, 0)
+            assert (got == want).all(), ("bfs", gi)
+            assert r == iters
+        elif algo == gdb.TASK_SSSP:
+            want = oracle.dijkstra_tb(g.row, g.col, g.w, 0)
+            assert (got.astype(np.int64) == want).all(), ("sssp", gi)
+        else:
+            nv = vals[gi].to_numpy()
+            want, rr = oracle.color_ref(g.row, g.col, nv)
+            assert (got == want).all(), ("color", gi)
+            assert r == rr
+
+
+@pytest.mark.parametrize("delta", [0, 64, 300])
+def test_batch_single_group_teams(ctx, delta):
+    gs = small_graphs(24, seed=4)
+    kinds, outs, vals, rounds, st = run_batch(ctx, gs, 1, delta)
+    print(st.as_dict())
+    check(gs, kinds, outs, vals, rounds)
+
+
+@pytest.mark.parametrize("team", [2, 8, 64])
+def test_batch_multi_group_teams(ctx, team):
+    gs = [gdb.gen_grid(200, 200, 0.05, seed=2, wlo=1, whi=255), gdb.gen_rmat(13, 16, seed=7, wlo=1, whi=255)]
+    kinds, outs, vals, rounds, st = run_batch(ctx, gs, team, 500, wgs=256, groups=4 * team)
+    print(team, st.as_dict())
+    check(gs, kinds, outs, vals, rounds)
+
+
+def test_batch_more_teams_than_tasks(ctx):
+    gs = small_graphs(2, seed=9)
+    kinds, outs, vals, rounds, st = run_batch(ctx, gs, 1, 100)
+    check(gs, kinds, outs, vals, rounds)
+
+
+def test_batch_matches_single_task_solvers(ctx):
+    g = gdb.gen_grid(128, 128, 0.05, seed=2, wlo=1, whi=255)
+    dg = gdb.DeviceCSR(g)
+    a, _ = gdb.sssp_worklist(ctx, dg, 0, gdb.Opts.make(variant=gdb.VARIANT_TUNED))
+    out = gdb.DeviceBuffer.empty(g.n, np.uint32)
+    t = gdb.make_task(gdb.TASK_SSSP, dg, out, 0, delta=1000)
+    gdb.batch_solve(ctx, [t], 16, gdb.Opts.make(wgs=512, groups=16))
+    assert (out.to_numpy() == a).all()
