@@ -102,6 +102,7 @@ def main() -> int:
 
     out_b = gdb.DeviceBuffer.empty(g.n, np.uint32)
     out_s = gdb.DeviceBuffer.empty(g.n, np.uint32)
+    last_tasks = []
 
     def solve_batched(wgs, groups, delta):
         # One launch, two teams: BFS and near/far SSSP side by side.
@@ -109,6 +110,7 @@ def main() -> int:
                  gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta)]
         rounds, st = gdb.batch_solve(ctx, tasks, max(1, groups // 2),
                                      gdb.Opts.make(wgs=wgs, groups=groups))
+        last_tasks[:] = tasks
         return rounds, st
 
     # Launch-shape / band-width autotuning of the batched engine (untimed).
@@ -117,9 +119,9 @@ def main() -> int:
     else:
         trials = []
         for w in (256, 512, 1024):
-            for gr in (16, 32, 64, 128, 256):
+            for gr in (16, 32, 64, 128):
                 gr = min(gr, gdb.max_resident_groups(w, 0))
-                for dl in (512, 2048):
+                for dl in (1024, 2048, 4096, 8192):
                     _, st = solve_batched(w, gr, dl)
                     trials.append((st.kernel_ms, w, gr, dl))
         trials.sort()
@@ -165,7 +167,7 @@ def main() -> int:
 
     # Roofline of the dominant kernel (the batched kernel): algorithmic bytes
     # = BFS 8 B per processed edge + 24 B per processed vertex, SSSP 16 B per
-    # processed edge + 24 B per processed worklist item (see DESIGN.md).
+    # processed edge + 24 B per processed worklist item.
     n_items, n_edges = bst.work_items, bst.work_edges
     alg_bytes = 16 * n_edges + 24 * n_items
     achieved = alg_bytes / (t_step_ms / 1e3) / 1e9
@@ -191,6 +193,7 @@ def main() -> int:
             "wgs": wgs, "groups": groups, "team_groups": max(1, groups // 2), "delta": delta,
             "participating": bst.participating, "bfs_levels": rounds[0], "sssp_iterations": rounds[1],
         },
+        "task_ms": {"bfs": round(last_tasks[0].time_us / 1e3, 3), "sssp": round(last_tasks[1].time_us / 1e3, 3)},
         "kernel_ms": round(t_step_ms, 3),
         "tuned_single": {"gteps": round(2 * mr / (tuned_ms / 1e3) / 1e9, 4), "ms": round(tuned_ms, 3)},
         "plain": {"gteps": round(plain_gteps, 4), "ms": round(plain_ms, 3),

@@ -88,6 +88,7 @@ class Task(C.Structure):
         ("out", C.c_void_p),
         ("delta", C.c_int),
         ("rounds", C.c_int),
+        ("time_us", C.c_double),
     ]
 
 
@@ -519,6 +520,8 @@ def batch_solve(ctx: Context, tasks: list, team_groups: int = 1, opts=None):
     o = _opts(opts)
     check(lib().gdb_batch_solve(ctx.h, arr, len(tasks), team_groups, C.byref(o), C.byref(st)),
           "gdb_batch_solve")
+    for i, t in enumerate(tasks):
+        t.rounds, t.time_us = arr[i].rounds, arr[i].time_us
     return [arr[i].rounds for i in range(len(tasks))], st
 
 

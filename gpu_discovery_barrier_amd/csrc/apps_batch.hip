@@ -24,6 +24,7 @@ namespace {
 
 constexpr unsigned kInf = 1000000000u;  // bfs_common.cl:35
 constexpr int kSerialLimit = 16;
+constexpr int kChunk = 4;                  // edges a lane relaxes per step
 constexpr int kLine = GDB_LINE_BYTES / 4;  // ints per 128 B line
 
 // Team state, one 128 B line per word.
@@ -52,14 +53,56 @@ struct Team {
   __device__ int stride() const { return G * (int)blockDim.x; }
 };
 
+__device__ inline int team_barrier_payload(Team& T, const int* src, int* lds_val);
+
+// Plain team barrier (payload unused).
 __device__ inline bool team_barrier(Team& T) {
+  __shared__ int dummy;
+  team_barrier_payload(T, T.word(TW_TASK), &dummy);
+  return T.l->aborted == 0;
+}
+
+// Team barrier that also hands every member one int: the value at *src read
+// by the last arriver, after every member's writes are visible to it (the
+// size of the queue the iteration just filled).  The generation and the
+// value travel in one 64-bit word, so a waiter gets both with one load.
+__device__ inline int team_barrier_payload(Team& T, const int* src, int* lds_val) {
   gdb::drain_stores();
   __syncthreads();
-  if (T.G == 1) return true;
-  if (threadIdx.x == 0)
-    gdb::counter_wait(T.ctx, T.word(TW_COUNT), T.word(TW_GEN), T.G, T.gen_cache, &T.l->aborted);
+  if (threadIdx.x == 0) {
+    if (T.G == 1) {
+      *lds_val = gdb::ld_rlx(src);
+    } else {
+      unsigned long long* word = (unsigned long long*)T.word(TW_GEN);
+      int g = *T.gen_cache;
+      if (g == GDB_GEN_UNKNOWN)
+        g = (int)(unsigned)__hip_atomic_load(word, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      const int old = gdb::fadd(T.word(TW_COUNT), 1);
+      int val = 0;
+      if (old == T.G - 1) {
+        gdb::st_rlx(T.word(TW_COUNT), 0);
+        val = gdb::ld_rlx(src);
+        const unsigned long long w = ((unsigned long long)(unsigned)val << 32) | (unsigned)(g + 1);
+        __hip_atomic_store(word, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        unsigned long long w = 0;
+        const bool ok = gdb::wait_while(
+            T.ctx,
+            [&] {
+              w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              return (int)(unsigned)w == g;
+            },
+            GDB_ERR_BARRIER_TIMEOUT);
+        gdb::fence_acq();
+        if (!ok) T.l->aborted = 1;
+        val = (int)(w >> 32);
+      }
+      *T.gen_cache = g + 1;
+      *lds_val = val;
+    }
+  }
   __syncthreads();
-  return T.l->aborted == 0;
+  return *lds_val;
 }
 
 template <bool WEIGHTED>
@@ -92,18 +135,29 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, long long v) {
 
 template <bool WEIGHTED>
 __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work) {
+  // scratch: BFS   q0[n], q1[n]
+  //          SSSP  d64[n] (8 B each), fstamp[n], q0[n], q1[n], far0..2[n]
+  // SSSP keeps (distance << 32 | mark) per vertex so that one 64-bit atomic
+  // min both lowers the distance and says whether the vertex was already
+  // queued this iteration: mark = 2*it for a near push, 2*it+1 for a far
+  // push; on equal distances the older mark stays (marks only grow).
+  __shared__ int lds_n;
   const int n = k.n;
   unsigned* dist = (unsigned*)k.out;
-  int* stamp = k.scratch;
-  int* fstamp = stamp + n;
-  int* q[2] = {fstamp + n, fstamp + 2 * n};
-  int* far[3] = {fstamp + 3 * n, fstamp + 4 * n, fstamp + 5 * n};
+  unsigned long long* d64 = (unsigned long long*)k.scratch;
+  int* base = WEIGHTED ? k.scratch + 2 * (size_t)n : k.scratch;
+  int* fstamp = base;
+  int* q[2] = {WEIGHTED ? base + n : base, WEIGHTED ? base + 2 * n : base + n};
+  int* far[3] = {base + 3 * n, base + 4 * n, base + 5 * n};
   const int tid = T.tid(), stride = T.stride();
   const bool near_far = WEIGHTED && k.delta > 0;
   for (int v = tid; v < n; v += stride) {
-    dist[v] = v == k.source ? 0u : kInf;
-    stamp[v] = 0;
-    fstamp[v] = -1;
+    if (WEIGHTED) {
+      d64[v] = (unsigned long long)(v == k.source ? 0u : kInf) << 32;
+      fstamp[v] = -1;
+    } else {
+      dist[v] = v == k.source ? 0u : kInf;
+    }
   }
   if (tid == 0) {
     q[0][0] = k.source;
@@ -125,6 +179,7 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
     int* outq = q[it & 1];
     int* out_tail = T.word(TW_TAIL0 + (it + 1) % 3);
     if (tid == 0) gdb::st_rlx(T.word(TW_TAIL0 + (it + 2) % 3), 0);
+    const unsigned near_mark = 2u * (unsigned)it, far_mark = near_mark + 1u;
     if (nin > 0) {
       if (nin > n) {
         gdb::set_error(T.ctx, GDB_ERR_QUEUE_OVERFLOW);
@@ -142,24 +197,59 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
         if (valid) {
           b = k.row[v];
           e = k.row[v + 1];
-          dv = WEIGHTED ? dist[v] : (unsigned)it;
+          dv = WEIGHTED ? (unsigned)(d64[v] >> 32) : (unsigned)it;
           ++items;
           edges += e - b;
         }
+        // Relaxes edge kk from a vertex at distance src_d; returns the push
+        // class: 0 none, 1 near, 2 far.
+        auto relax = [&](int kk, unsigned src_d, int* dst_out) -> int {
+          const int dst = k.col[kk];
+          *dst_out = dst;
+          if ((unsigned)dst >= (unsigned)n) return 0;
+          if (!WEIGHTED) return atomicCAS(&dist[dst], kInf, src_d) == kInf ? 1 : 0;
+          const unsigned wt = (unsigned)k.w[kk];
+          if (wt >= kInf) return 0;
+          const unsigned cand = src_d + wt;
+          const bool nearp = cand < thresh;
+          const unsigned long long nw =
+              ((unsigned long long)cand << 32) | (nearp ? near_mark : far_mark);
+          const unsigned long long old = atomicMin(&d64[dst], nw);
+          if (cand >= (unsigned)(old >> 32)) return 0;
+          if (nearp) return (unsigned)old == near_mark ? 0 : 1;
+          return atomicExch(&fstamp[dst], gen) != gen ? 2 : 0;
+        };
+        // Light rows: each lane relaxes its own edges four at a time, the
+        // four atomics of a chunk issued back to back.
+        const bool heavy = valid && (e - b) > kSerialLimit;
+        const int len = (valid && !heavy) ? (e - b) : 0;
+        const int maxlen = gdb::wave_max(len);
+        for (int c = 0; c < maxlen; c += kChunk) {
+          int dst[kChunk], cls[kChunk];
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) {
+            dst[j] = -1;
+            cls[j] = (c + j < len) ? relax(b + c + j, dv, &dst[j]) : 0;
+          }
+          bool to_near[kChunk], to_far[kChunk];
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) {
+            to_near[j] = cls[j] == 1;
+            to_far[j] = cls[j] == 2;
+          }
+          gdb::wave_push_n<kChunk>(to_near, dst, outq, out_tail);
+          if (near_far) gdb::wave_push_n<kChunk>(to_far, dst, fpile, ftail);
+        }
+        // Heavy rows: the whole wave walks one row, 64 edges per step.
         gdb::wave_coop_foreach(
-            valid != 0, b, e, dv,
+            heavy, b, e, dv,
             [&](int kk, bool ok, unsigned src_d) {
-              int dst = 0;
-              const unsigned nd = ok ? relax_target<WEIGHTED>(k.col, k.w, dist, kk, src_d, n, &dst) : kInf;
-              bool to_near = false, to_far = false;
-              if (nd != kInf) {
-                if (nd < thresh) to_near = atomicExch(&stamp[dst], it) != it;
-                else to_far = atomicExch(&fstamp[dst], gen) != gen;
-              }
-              gdb::wave_push(to_near, dst, outq, out_tail);
-              if (near_far) gdb::wave_push(to_far, dst, fpile, ftail);
+              int d2 = 0;
+              const int cls = ok ? relax(kk, src_d, &d2) : 0;
+              gdb::wave_push(cls == 1, d2, outq, out_tail);
+              if (near_far) gdb::wave_push(cls == 2, d2, fpile, ftail);
             },
-            kSerialLimit);
+            -1);
       }
     } else {
       // Near queue empty: split the far pile of generation `gen` at the next
@@ -179,9 +269,9 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
         int v = -1;
         bool to_near = false, to_far = false;
         if (valid) {
-          v = src_pile[i];
+          v = src_pile[i];  // each vertex at most once per generation
           if ((unsigned)v < (unsigned)n) {
-            if (dist[v] < thresh) to_near = atomicExch(&stamp[v], it) != it;
+            if ((unsigned)(d64[v] >> 32) < thresh) to_near = true;
             else to_far = atomicExch(&fstamp[v], gen + 1) != gen + 1;
           }
         }
@@ -190,10 +280,12 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
       }
       ++gen;
     }
-    if (!team_barrier(T)) return -1;
-    nin = gdb::ld_rlx(out_tail);
+    nin = team_barrier_payload(T, out_tail, &lds_n);
+    if (T.l->aborted) return -1;
     ++it;
   }
+  if (WEIGHTED)
+    for (int v = tid; v < n; v += stride) dist[v] = (unsigned)(d64[v] >> 32);
   wave_add(&work[0], items);
   wave_add(&work[1], edges);
   return it - 1;
@@ -287,14 +379,18 @@ __global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevT
     const int t = gdb::ld_rlx(T.word(TW_TASK));
     if (t >= ntasks) break;
     const DevTask k = tasks[t];
+    if (T.tid() == 0) work[4 * (size_t)t + 2] = gdb::now();
     int rounds = 0;
     if (k.n > 0) {
-      if (k.algo == GDB_TASK_BFS) rounds = team_worklist<false>(T, k, work + 2 * (size_t)t);
-      else if (k.algo == GDB_TASK_SSSP) rounds = team_worklist<true>(T, k, work + 2 * (size_t)t);
+      if (k.algo == GDB_TASK_BFS) rounds = team_worklist<false>(T, k, work + 4 * (size_t)t);
+      else if (k.algo == GDB_TASK_SSSP) rounds = team_worklist<true>(T, k, work + 4 * (size_t)t);
       else rounds = team_color(T, k);
     }
     if (rounds < 0) return;
-    if (T.tid() == 0) results[t] = rounds;
+    if (T.tid() == 0) {
+      results[t] = rounds;
+      work[4 * (size_t)t + 3] = gdb::now();
+    }
     if (!team_barrier(T)) return;
   }
 }
@@ -325,7 +421,7 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
         (a.algo == GDB_TASK_COLOR && a.n > 0 && !a.node_value) ||
         (a.algo != GDB_TASK_COLOR && a.n > 0 && (a.source < 0 || a.source >= a.n)) || a.delta < 0)
       return GDB_E_INVALID;
-    const size_t ints = a.algo == GDB_TASK_COLOR ? (size_t)2 * a.n : (size_t)7 * a.n;
+    const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP ? 8 : 2) * a.n;
     dt[t] = DevTask{a.algo, a.n, a.m, a.source, a.delta, (int)0, a.row, a.col, a.w, a.node_value, a.out,
                     nullptr};
     scratch_total += (ints + 63) & ~(size_t)63;
@@ -339,7 +435,7 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
   if (rc) return rc;
   const size_t team_ints = (size_t)groups * TW_WORDS * kLine;
   const size_t bytes = carve_bytes<DevTask>(ntasks) + carve_bytes<int>(team_ints) +
-                       carve_bytes<int>(ntasks + 64) + carve_bytes<unsigned long long>(2 * (size_t)ntasks) +
+                       carve_bytes<int>(ntasks + 64) + carve_bytes<unsigned long long>(4 * (size_t)ntasks) +
                        carve_bytes<int>(scratch_total);
   rc = ensure_workspace(ctx, bytes);
   if (rc) return rc;
@@ -347,18 +443,18 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
   DevTask* d_tasks = cv.take<DevTask>(ntasks);
   int* team_state = cv.take<int>(team_ints);
   int* misc = cv.take<int>(ntasks + 64);
-  unsigned long long* work = cv.take<unsigned long long>(2 * (size_t)ntasks);
+  unsigned long long* work = cv.take<unsigned long long>(4 * (size_t)ntasks);
   int* scratch = cv.take<int>(scratch_total);
   size_t off = 0;
   for (int t = 0; t < ntasks; ++t) {
-    const size_t ints = dt[t].algo == GDB_TASK_COLOR ? (size_t)2 * dt[t].n : (size_t)7 * dt[t].n;
+    const size_t ints = (size_t)(dt[t].algo == GDB_TASK_SSSP ? 8 : 2) * dt[t].n;
     dt[t].scratch = scratch + off;
     off += (ints + 63) & ~(size_t)63;
   }
   GDB_HIP(hipMemcpyAsync(d_tasks, dt.data(), sizeof(DevTask) * ntasks, hipMemcpyHostToDevice, ctx->stream));
   rc = fill_i32(ctx, team_state, 0, (long long)team_ints);
   if (!rc) rc = fill_i32(ctx, misc, 0, ntasks + 64);
-  if (!rc) rc = fill_i32(ctx, (int*)work, 0, 4 * (long long)ntasks);
+  if (!rc) rc = fill_i32(ctx, (int*)work, 0, 8 * (long long)ntasks);
   if (rc) return rc;
   int* next_task = misc;
   int* teams_used = misc + 32;
@@ -377,17 +473,19 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
   });
   if (rc) return rc;
   std::vector<int> hres((size_t)ntasks);
-  std::vector<unsigned long long> hwork((size_t)2 * ntasks);
+  std::vector<unsigned long long> hwork((size_t)4 * ntasks);
   GDB_HIP(hipMemcpy(hres.data(), results, sizeof(int) * ntasks, hipMemcpyDeviceToHost));
-  GDB_HIP(hipMemcpy(hwork.data(), work, sizeof(unsigned long long) * 2 * ntasks, hipMemcpyDeviceToHost));
+  GDB_HIP(hipMemcpy(hwork.data(), work, sizeof(unsigned long long) * 4 * ntasks, hipMemcpyDeviceToHost));
   for (int t = 0; t < ntasks; ++t) {
     tasks[t].rounds = hres[t];
     st.rounds += hres[t];
     // Worklist tasks report items / edges processed; colour tasks count the
     // graph once.
     const bool wl = tasks[t].algo != GDB_TASK_COLOR;
-    st.work_items += wl ? (long long)hwork[2 * t] : tasks[t].n;
-    st.work_edges += wl ? (long long)hwork[2 * t + 1] : tasks[t].m;
+    st.work_items += wl ? (long long)hwork[4 * t] : tasks[t].n;
+    st.work_edges += wl ? (long long)hwork[4 * t + 1] : tasks[t].m;
+    // Device wall clock (100 MHz) from the team's start to end of the task.
+    tasks[t].time_us = (double)(hwork[4 * t + 3] - hwork[4 * t + 2]) / 100.0;
   }
   st.launches = 1;
   rc = read_participating(ctx, &st.participating);

@@ -91,6 +91,39 @@ __device__ __forceinline__ void wave_push(bool push, int item, int* queue, int* 
   if (push) queue[base + __popcll(m & ((1ull << lane) - 1ull))] = item;
 }
 
+// Appends the items of N push slots of every lane with one atomic per wave.
+template <int N>
+__device__ __forceinline__ void wave_push_n(const bool (&push)[N], const int (&item)[N], int* queue,
+                                            int* tail) {
+  unsigned long long m[N];
+  int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    m[j] = __ballot(push[j]);
+    cnt += __popcll(m[j]);
+  }
+  if (cnt == 0) return;
+  const int lane = lane_id();
+  int base = 0;
+  if (lane == 0) base = fadd_rlx(tail, cnt);
+  base = __shfl(base, 0, 64);
+  const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    if (push[j]) queue[base + __popcll(m[j] & lt)] = item[j];
+    base += __popcll(m[j]);
+  }
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 // Wave-wide OR of a predicate.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 

@@ -193,6 +193,7 @@ typedef struct {
   void* out;                /* device output: u32 distances or i32 colours */
   int delta;                /* SSSP near-far band width (0 = plain worklist) */
   int rounds;               /* out: levels / iterations / colour rounds */
+  double time_us;           /* out: device time from task start to task end */
 } gdb_task;
 
 /* team_groups: groups per team (0 = 1).  opts->wgs and opts->groups pick the
