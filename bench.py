@@ -104,10 +104,10 @@ def main() -> int:
     out_s = gdb.DeviceBuffer.empty(g.n, np.uint32)
     last_tasks = []
 
-    def solve_batched(wgs, groups, delta):
+    def solve_batched(wgs, groups, delta, hops=0):
         # One launch, two teams: BFS and near/far SSSP side by side.
-        tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0),
-                 gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta)]
+        tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0, hops=hops),
+                 gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta, hops=hops)]
         rounds, st = gdb.batch_solve(ctx, tasks, max(1, groups // 2),
                                      gdb.Opts.make(wgs=wgs, groups=groups))
         last_tasks[:] = tasks
@@ -115,26 +115,27 @@ def main() -> int:
 
     # Launch-shape / band-width autotuning of the batched engine (untimed).
     if args.no_tune:
-        wgs, groups, delta = 256, 64, 1024
+        wgs, groups, delta, hops = 256, 64, 4096, 8
     else:
         trials = []
         for w in (256, 512, 1024):
             for gr in (16, 32, 64, 128):
                 gr = min(gr, gdb.max_resident_groups(w, 0))
-                for dl in (1024, 2048, 4096, 8192):
-                    _, st = solve_batched(w, gr, dl)
-                    trials.append((st.kernel_ms, w, gr, dl))
+                for dl in (2048, 4096, 8192):
+                    for hp in (0, 4, 16):
+                        _, st = solve_batched(w, gr, dl, hp)
+                        trials.append((st.kernel_ms, w, gr, dl, hp))
         trials.sort()
-        _, wgs, groups, delta = trials[0]
-        log(f"autotune: wgs={wgs} groups={groups} delta={delta} ({trials[0][0]:.2f} ms); "
+        _, wgs, groups, delta, hops = trials[0]
+        log(f"autotune: wgs={wgs} groups={groups} delta={delta} hops={hops} ({trials[0][0]:.2f} ms); "
             f"{len(trials)} shapes tried")
 
     for _ in range(args.warmup):
-        solve_batched(wgs, groups, delta)
+        solve_batched(wgs, groups, delta, hops)
     step_ms, walls = [], []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        rounds, bst = solve_batched(wgs, groups, delta)
+        rounds, bst = solve_batched(wgs, groups, delta, hops)
         walls.append(time.perf_counter() - t0)
         step_ms.append(bst.kernel_ms)
     db, ds = out_b.to_numpy(), out_s.to_numpy()
@@ -191,7 +192,8 @@ def main() -> int:
             "n": g.n, "m": g.m, "m_reached": mr,
             "engine": "batched (BFS and near/far SSSP teams in one persistent launch)",
             "wgs": wgs, "groups": groups, "team_groups": max(1, groups // 2), "delta": delta,
-            "participating": bst.participating, "bfs_levels": rounds[0], "sssp_iterations": rounds[1],
+            "local_hops": hops,
+            "participating": bst.participating, "bfs_iterations": rounds[0], "sssp_iterations": rounds[1],
         },
         "task_ms": {"bfs": round(last_tasks[0].time_us / 1e3, 3), "sssp": round(last_tasks[1].time_us / 1e3, 3)},
         "kernel_ms": round(t_step_ms, 3),

@@ -87,6 +87,7 @@ class Task(C.Structure):
         ("node_value", C.c_void_p),
         ("out", C.c_void_p),
         ("delta", C.c_int),
+        ("hops", C.c_int),
         ("rounds", C.c_int),
         ("time_us", C.c_double),
     ]
@@ -500,7 +501,7 @@ def verify_distances(ctx: Context, g: DeviceCSR, dist: DeviceBuffer, weighted: b
 # Batched engine
 # ---------------------------------------------------------------------------
 def make_task(algo: int, g: "DeviceCSR", out: DeviceBuffer, source: int = 0,
-              node_value: DeviceBuffer | None = None, delta: int = 0) -> Task:
+              node_value: DeviceBuffer | None = None, delta: int = 0, hops: int = 0) -> Task:
     t = Task()
     t.algo, t.n, t.m = algo, g.n, g.m
     t.row, t.col = g.row.ptr, g.col.ptr
@@ -509,6 +510,7 @@ def make_task(algo: int, g: "DeviceCSR", out: DeviceBuffer, source: int = 0,
     t.node_value = node_value.ptr if node_value is not None else None
     t.out = out.ptr
     t.delta = delta
+    t.hops = hops
     t.rounds = 0
     return t
 

@@ -32,7 +32,7 @@ enum TeamWord { TW_COUNT = 0, TW_GEN, TW_TASK, TW_TAIL0, TW_TAIL1, TW_TAIL2, TW_
                 TW_STOP0, TW_STOP1, TW_STOP2, TW_WORDS };
 
 struct DevTask {
-  int algo, n, m, source, delta, scratch_ints;
+  int algo, n, m, source, delta, hops;
   const int* row;
   const int* col;
   const int* w;
@@ -41,12 +41,16 @@ struct DevTask {
   int* scratch;
 };
 
+constexpr int kLocalQ = 2048;  // LDS queue capacity per buffer (ints)
+
 struct Team {
   gdb_kernel_ctx_t* ctx;
   gdb_local_ctx* l;
   int* ts;        // this team's state lines
   int* gen_cache; // LDS
   int G, rank;
+  int (*lq)[kLocalQ];  // LDS: two local queue buffers
+  int* lcnt;           // LDS: three rotating local queue counters
   __device__ int word_off(int w) const { return w * kLine; }
   __device__ int* word(int w) const { return ts + w * kLine; }
   __device__ int tid() const { return rank * (int)blockDim.x + (int)threadIdx.x; }
@@ -133,28 +137,46 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, long long v) {
   if (gdb::lane_id() == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
-template <bool WEIGHTED>
+// Worklist BFS / SSSP on a team.
+//   WEIGHTED = false : level-synchronous BFS, vertices claimed by CAS with
+//                      dist = iteration (the reference's processedge2).
+//   WEIGHTED = true  : label-correcting relaxation with a 64-bit
+//                      (distance << 32 | mark) word per vertex; UNIT = true
+//                      gives BFS levels (unit weights).
+// With k.hops = H > 0 a group, after its share of the iteration's queue,
+// keeps relaxing the vertices it improved through an LDS queue for up to H
+// local steps (workgroup barriers only), so one team barrier covers several
+// hops; what is left spills to the team's queue.  Any relaxation order
+// reaches the same fixpoint (exact levels / distances).
+template <bool WEIGHTED, bool UNIT>
 __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work) {
-  // scratch: BFS   q0[n], q1[n]
-  //          SSSP  d64[n] (8 B each), fstamp[n], q0[n], q1[n], far0..2[n]
-  // SSSP keeps (distance << 32 | mark) per vertex so that one 64-bit atomic
-  // min both lowers the distance and says whether the vertex was already
-  // queued this iteration: mark = 2*it for a near push, 2*it+1 for a far
-  // push; on equal distances the older mark stays (marks only grow).
+  // scratch: BFS (CAS)  q0[n], q1[n]
+  //          otherwise  d64[n] (8 B each), fstamp[n], gstamp[n], q0[n], q1[n], far0..2[n]
+  // Marks in the low word of d64 (level-synchronous iterations only):
+  // 2*it for a near push of iteration it, 2*it+1 for a far push; on equal
+  // distances the older mark stays, so "already queued in this iteration"
+  // is one compare.  Iterations with local hops leave the mark at 0 and
+  // dedupe team-queue pushes with gstamp instead, because a vertex pushed
+  // to an LDS queue may be relaxed again before the iteration ends.
   __shared__ int lds_n;
+  int (*lq)[kLocalQ] = T.lq;
+  int* lcnt = T.lcnt;
   const int n = k.n;
   unsigned* dist = (unsigned*)k.out;
   unsigned long long* d64 = (unsigned long long*)k.scratch;
   int* base = WEIGHTED ? k.scratch + 2 * (size_t)n : k.scratch;
   int* fstamp = base;
-  int* q[2] = {WEIGHTED ? base + n : base, WEIGHTED ? base + 2 * n : base + n};
-  int* far[3] = {base + 3 * n, base + 4 * n, base + 5 * n};
+  int* gstamp = base + n;
+  int* q[2] = {WEIGHTED ? base + 2 * (size_t)n : base, WEIGHTED ? base + 3 * (size_t)n : base + n};
+  int* far[3] = {base + 4 * (size_t)n, base + 5 * (size_t)n, base + 6 * (size_t)n};
   const int tid = T.tid(), stride = T.stride();
-  const bool near_far = WEIGHTED && k.delta > 0;
+  const bool near_far = WEIGHTED && !UNIT && k.delta > 0;
+  const int hops = WEIGHTED ? k.hops : 0;
   for (int v = tid; v < n; v += stride) {
     if (WEIGHTED) {
       d64[v] = (unsigned long long)(v == k.source ? 0u : kInf) << 32;
       fstamp[v] = -1;
+      gstamp[v] = 0;
     } else {
       dist[v] = v == k.source ? 0u : kInf;
     }
@@ -187,69 +209,116 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
       }
       int* fpile = far[gen % 3];
       int* ftail = T.word(TW_FAR0 + gen % 3);
-      GDB_FOR_UNIFORM(i, valid, tid, stride, nin) {
-        int v = -1, b = 0, e = 0;
-        unsigned dv = 0;
-        if (valid) {
-          v = in[i];
-          if ((unsigned)v >= (unsigned)n) valid = 0;
-        }
-        if (valid) {
-          b = k.row[v];
-          e = k.row[v + 1];
-          dv = WEIGHTED ? (unsigned)(d64[v] >> 32) : (unsigned)it;
-          ++items;
-          edges += e - b;
-        }
-        // Relaxes edge kk from a vertex at distance src_d; returns the push
-        // class: 0 none, 1 near, 2 far.
-        auto relax = [&](int kk, unsigned src_d, int* dst_out) -> int {
-          const int dst = k.col[kk];
-          *dst_out = dst;
-          if ((unsigned)dst >= (unsigned)n) return 0;
-          if (!WEIGHTED) return atomicCAS(&dist[dst], kInf, src_d) == kInf ? 1 : 0;
-          const unsigned wt = (unsigned)k.w[kk];
-          if (wt >= kInf) return 0;
-          const unsigned cand = src_d + wt;
-          const bool nearp = cand < thresh;
-          const unsigned long long nw =
-              ((unsigned long long)cand << 32) | (nearp ? near_mark : far_mark);
-          const unsigned long long old = atomicMin(&d64[dst], nw);
+      if (hops > 0 && threadIdx.x < 3) lcnt[threadIdx.x] = 0;
+      if (hops > 0) __syncthreads();
+      // Relaxes edge kk from a vertex at distance src_d.  Returns the push
+      // class: 0 none, 1 team queue, 2 far pile, 3 local (LDS) queue.
+      auto relax = [&](int kk, unsigned src_d, int* dst_out) -> int {
+        const int dst = k.col[kk];
+        *dst_out = dst;
+        if ((unsigned)dst >= (unsigned)n) return 0;
+        if (!WEIGHTED) return atomicCAS(&dist[dst], kInf, src_d) == kInf ? 1 : 0;
+        const unsigned wt = UNIT ? 1u : (unsigned)k.w[kk];
+        if (wt >= kInf) return 0;
+        const unsigned cand = src_d + wt;
+        const bool nearp = cand < thresh;
+        if (hops > 0) {
+          const unsigned long long old = atomicMin(&d64[dst], (unsigned long long)cand << 32);
           if (cand >= (unsigned)(old >> 32)) return 0;
-          if (nearp) return (unsigned)old == near_mark ? 0 : 1;
+          if (nearp) return 3;
           return atomicExch(&fstamp[dst], gen) != gen ? 2 : 0;
-        };
-        // Light rows: each lane relaxes its own edges four at a time, the
-        // four atomics of a chunk issued back to back.
-        const bool heavy = valid && (e - b) > kSerialLimit;
-        const int len = (valid && !heavy) ? (e - b) : 0;
-        const int maxlen = gdb::wave_max(len);
-        for (int c = 0; c < maxlen; c += kChunk) {
-          int dst[kChunk], cls[kChunk];
-#pragma unroll
-          for (int j = 0; j < kChunk; ++j) {
-            dst[j] = -1;
-            cls[j] = (c + j < len) ? relax(b + c + j, dv, &dst[j]) : 0;
-          }
-          bool to_near[kChunk], to_far[kChunk];
-#pragma unroll
-          for (int j = 0; j < kChunk; ++j) {
-            to_near[j] = cls[j] == 1;
-            to_far[j] = cls[j] == 2;
-          }
-          gdb::wave_push_n<kChunk>(to_near, dst, outq, out_tail);
-          if (near_far) gdb::wave_push_n<kChunk>(to_far, dst, fpile, ftail);
         }
-        // Heavy rows: the whole wave walks one row, 64 edges per step.
-        gdb::wave_coop_foreach(
-            heavy, b, e, dv,
-            [&](int kk, bool ok, unsigned src_d) {
-              int d2 = 0;
-              const int cls = ok ? relax(kk, src_d, &d2) : 0;
-              gdb::wave_push(cls == 1, d2, outq, out_tail);
-              if (near_far) gdb::wave_push(cls == 2, d2, fpile, ftail);
-            },
-            -1);
+        const unsigned long long nw = ((unsigned long long)cand << 32) | (nearp ? near_mark : far_mark);
+        const unsigned long long old = atomicMin(&d64[dst], nw);
+        if (cand >= (unsigned)(old >> 32)) return 0;
+        if (nearp) return (unsigned)old == near_mark ? 0 : 1;
+        return atomicExch(&fstamp[dst], gen) != gen ? 2 : 0;
+      };
+      // Pushes the results of one relaxation step of this wave.  Class 3
+      // goes to LDS buffer `lb` when there is room, else to the team queue.
+      auto push_n = [&](const int (&dst)[kChunk], const int (&cls)[kChunk], int lb, int* lc) {
+        bool to_team[kChunk], to_far[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          bool team_push = cls[j] == 1;
+          if (cls[j] == 3) {
+            const int pos = atomicAdd(lc, 1);
+            if (pos < kLocalQ) lq[lb][pos] = dst[j];
+            else team_push = atomicExch(&gstamp[dst[j]], it) != it;
+          }
+          to_team[j] = team_push;
+          to_far[j] = cls[j] == 2;
+        }
+        gdb::wave_push_n<kChunk>(to_team, dst, outq, out_tail);
+        if (near_far) gdb::wave_push_n<kChunk>(to_far, dst, fpile, ftail);
+      };
+      // Relaxes the out-edges of items[start, count) with stride `step`.
+      auto process = [&](const int* src_items, int count, int start, int step, int lb, int* lc) {
+        GDB_FOR_UNIFORM(i, valid, start, step, count) {
+          int v = -1, b = 0, e = 0;
+          unsigned dv = 0;
+          if (valid) {
+            v = src_items[i];
+            if ((unsigned)v >= (unsigned)n) valid = 0;
+          }
+          if (valid) {
+            b = k.row[v];
+            e = k.row[v + 1];
+            dv = WEIGHTED ? (unsigned)(d64[v] >> 32) : (unsigned)it;
+            ++items;
+            edges += e - b;
+          }
+          // Light rows: each lane relaxes its own edges four at a time, the
+          // four atomics of a chunk issued back to back.
+          const bool heavy = valid && (e - b) > kSerialLimit;
+          const int len = (valid && !heavy) ? (e - b) : 0;
+          const int maxlen = gdb::wave_max(len);
+          for (int c = 0; c < maxlen; c += kChunk) {
+            int dst[kChunk], cls[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+              dst[j] = -1;
+              cls[j] = (c + j < len) ? relax(b + c + j, dv, &dst[j]) : 0;
+            }
+            push_n(dst, cls, lb, lc);
+          }
+          // Heavy rows: the whole wave walks one row, 64 edges per step.
+          gdb::wave_coop_foreach(
+              heavy, b, e, dv,
+              [&](int kk, bool ok, unsigned src_d) {
+                int dst[kChunk] = {-1, -1, -1, -1}, cls[kChunk] = {0, 0, 0, 0};
+                cls[0] = ok ? relax(kk, src_d, &dst[0]) : 0;
+                push_n(dst, cls, lb, lc);
+              },
+              -1);
+        }
+      };
+      // The team's share of the iteration; improved vertices go to LDS
+      // buffer 1 (local step 1 reads it).
+      process(in, nin, tid, stride, 1, &lcnt[1]);
+      // Local steps: this group alone relaxes what it just improved.
+      int s = 1;
+      for (; s <= hops; ++s) {
+        gdb::drain_stores();
+        __syncthreads();
+        const int ln = min(lcnt[s % 3], kLocalQ);
+        if (ln == 0) break;
+        if (threadIdx.x == 0) lcnt[(s + 2) % 3] = 0;
+        process(lq[s & 1], ln, (int)threadIdx.x, (int)blockDim.x, (s + 1) & 1, &lcnt[(s + 1) % 3]);
+      }
+      if (hops > 0 && s > hops) {
+        // Budget spent: flush the last local buffer to the team queue.
+        __syncthreads();
+        const int ln = min(lcnt[s % 3], kLocalQ);
+        GDB_FOR_UNIFORM(i, valid, (int)threadIdx.x, (int)blockDim.x, ln) {
+          int v = 0;
+          bool p = false;
+          if (valid) {
+            v = lq[s & 1][i];
+            p = atomicExch(&gstamp[v], it) != it;
+          }
+          gdb::wave_push(p, v, outq, out_tail);
+        }
       }
     } else {
       // Near queue empty: split the far pile of generation `gen` at the next
@@ -271,7 +340,7 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
         if (valid) {
           v = src_pile[i];  // each vertex at most once per generation
           if ((unsigned)v < (unsigned)n) {
-            if ((unsigned)(d64[v] >> 32) < thresh) to_near = true;
+            if ((unsigned)(d64[v] >> 32) < thresh) to_near = atomicExch(&gstamp[v], it) != it;
             else to_far = atomicExch(&fstamp[v], gen + 1) != gen + 1;
           }
         }
@@ -363,6 +432,8 @@ __global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevT
                                                unsigned long long* work) {
   DISCOVERY_PROTOCOL(ctx);
   __shared__ int gen_cache;
+  __shared__ int lq[2][kLocalQ];
+  __shared__ int lcnt[3];
   const int P = p_get_num_groups(ctx, &local_ctx);
   const int id = p_get_group_id(ctx, &local_ctx);
   if (G > P) G = P;
@@ -371,7 +442,7 @@ __global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevT
   const int team = id / G;
   if (team >= nteams) return;
   if (threadIdx.x == 0) gen_cache = GDB_GEN_UNKNOWN;
-  Team T{ctx, &local_ctx, team_state + (size_t)team * TW_WORDS * kLine, &gen_cache, G, id % G};
+  Team T{ctx, &local_ctx, team_state + (size_t)team * TW_WORDS * kLine, &gen_cache, G, id % G, lq, lcnt};
   if (id == 0 && threadIdx.x == 0) *teams_used = nteams;
   for (;;) {
     if (T.rank == 0 && threadIdx.x == 0) gdb::st_rlx(T.word(TW_TASK), gdb::fadd_rlx(next_task, 1));
@@ -382,8 +453,10 @@ __global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevT
     if (T.tid() == 0) work[4 * (size_t)t + 2] = gdb::now();
     int rounds = 0;
     if (k.n > 0) {
-      if (k.algo == GDB_TASK_BFS) rounds = team_worklist<false>(T, k, work + 4 * (size_t)t);
-      else if (k.algo == GDB_TASK_SSSP) rounds = team_worklist<true>(T, k, work + 4 * (size_t)t);
+      unsigned long long* wk = work + 4 * (size_t)t;
+      if (k.algo == GDB_TASK_BFS)
+        rounds = k.hops > 0 ? team_worklist<true, true>(T, k, wk) : team_worklist<false, false>(T, k, wk);
+      else if (k.algo == GDB_TASK_SSSP) rounds = team_worklist<true, false>(T, k, wk);
       else rounds = team_color(T, k);
     }
     if (rounds < 0) return;
@@ -419,10 +492,11 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
     if (a.n < 0 || a.m < 0 || a.algo < GDB_TASK_BFS || a.algo > GDB_TASK_COLOR || !a.out ||
         (a.n > 0 && !a.row) || (a.m > 0 && !a.col) || (a.algo == GDB_TASK_SSSP && a.m > 0 && !a.w) ||
         (a.algo == GDB_TASK_COLOR && a.n > 0 && !a.node_value) ||
-        (a.algo != GDB_TASK_COLOR && a.n > 0 && (a.source < 0 || a.source >= a.n)) || a.delta < 0)
+        (a.algo != GDB_TASK_COLOR && a.n > 0 && (a.source < 0 || a.source >= a.n)) || a.delta < 0 ||
+        a.hops < 0 || a.hops > 64)
       return GDB_E_INVALID;
-    const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP ? 8 : 2) * a.n;
-    dt[t] = DevTask{a.algo, a.n, a.m, a.source, a.delta, (int)0, a.row, a.col, a.w, a.node_value, a.out,
+    const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : 2) * a.n;
+    dt[t] = DevTask{a.algo, a.n, a.m, a.source, a.delta, a.hops, a.row, a.col, a.w, a.node_value, a.out,
                     nullptr};
     scratch_total += (ints + 63) & ~(size_t)63;
     if (a.n > max_n) max_n = a.n;
@@ -447,7 +521,7 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
   int* scratch = cv.take<int>(scratch_total);
   size_t off = 0;
   for (int t = 0; t < ntasks; ++t) {
-    const size_t ints = (size_t)(dt[t].algo == GDB_TASK_SSSP ? 8 : 2) * dt[t].n;
+    const size_t ints = (size_t)(dt[t].algo == GDB_TASK_SSSP || dt[t].hops > 0 ? 9 : 2) * dt[t].n;
     dt[t].scratch = scratch + off;
     off += (ints + 63) & ~(size_t)63;
   }

@@ -192,6 +192,8 @@ typedef struct {
   const float* node_value;  /* colour priorities */
   void* out;                /* device output: u32 distances or i32 colours */
   int delta;                /* SSSP near-far band width (0 = plain worklist) */
+  int hops;                 /* BFS / SSSP: local hops a group may run past its share of
+                               an iteration before the team barrier (0 = level-synchronous) */
   int rounds;               /* out: levels / iterations / colour rounds */
   double time_us;           /* out: device time from task start to task end */
 } gdb_task;
