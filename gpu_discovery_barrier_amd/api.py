@@ -88,8 +88,34 @@ class Task(C.Structure):
         ("out", C.c_void_p),
         ("delta", C.c_int),
         ("hops", C.c_int),
+        ("trow", C.c_void_p),
+        ("tcol", C.c_void_p),
+        ("tw", C.c_void_p),
         ("rounds", C.c_int),
         ("time_us", C.c_double),
+        ("arm_iters", C.c_int * 4),
+    ]
+
+
+# Schedule selector (gdb_selector): modes and arms.
+SEL_NONE, SEL_FIXED, SEL_HEURISTIC, SEL_RANDOM, SEL_MLP = 0, 1, 2, 3, 4
+ARM_PUSH, ARM_PULL, ARM_PUSH_NARROW, ARM_PULL_NARROW = 0, 1, 2, 3
+MLP_IN, MLP_HIDDEN, MLP_OUT = 16, 64, 4
+MLP_PARAMS = MLP_IN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_OUT + MLP_OUT
+TRACE_STRIDE = 20
+
+
+class Selector(C.Structure):
+    """gdb_selector (include/gdb_amd.h)."""
+    _fields_ = [
+        ("mode", C.c_int),
+        ("fixed_arm", C.c_int),
+        ("narrow_groups", C.c_int),
+        ("seed", C.c_uint),
+        ("weights", C.c_void_p),
+        ("trace", C.c_void_p),
+        ("trace_capacity", C.c_int),
+        ("trace_count", C.c_void_p),
     ]
 
 
@@ -152,6 +178,12 @@ def _declare(lib) -> None:
         "gdb_sssp_worklist": (I, [P, I, I, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_verify_distances": (I, [P, I, I, P, P, P, P, C.POINTER(C.c_longlong)]),
         "gdb_batch_solve": (I, [P, C.POINTER(Task), I, I, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_batch_solve_sel": (I, [P, C.POINTER(Task), I, I, C.POINTER(Opts), C.POINTER(Selector),
+                                    C.POINTER(Stats)]),
+        "gdb_mlp_forward": (I, [P, P, P, I, P]),
+        "gdb_mlp_grad": (I, [P, P, P, I, P, C.POINTER(C.c_float), C.POINTER(I)]),
+        "gdb_mlp_to_bf16": (I, [P, P, P, I]),
+        "gdb_mlp_train": (I, [P, P, P, P, P, P, I, I, I, C.c_float, C.c_uint, C.POINTER(C.c_float)]),
         "gdb_load_metis": (I, [C.c_char_p, I, C.POINTER(HostCSRStruct)]),
         "gdb_load_dimacs": (I, [C.c_char_p, I, I, C.POINTER(HostCSRStruct)]),
         "gdb_load_galois_gr": (I, [C.c_char_p, C.POINTER(HostCSRStruct)]),
@@ -501,7 +533,10 @@ def verify_distances(ctx: Context, g: DeviceCSR, dist: DeviceBuffer, weighted: b
 # Batched engine
 # ---------------------------------------------------------------------------
 def make_task(algo: int, g: "DeviceCSR", out: DeviceBuffer, source: int = 0,
-              node_value: DeviceBuffer | None = None, delta: int = 0, hops: int = 0) -> Task:
+              node_value: DeviceBuffer | None = None, delta: int = 0, hops: int = 0,
+              transpose: "DeviceCSR | None" = None) -> Task:
+    """`transpose` is the in-edge CSR (the graph itself when it is symmetric);
+    it enables the pull arms of the schedule selector."""
     t = Task()
     t.algo, t.n, t.m = algo, g.n, g.m
     t.row, t.col = g.row.ptr, g.col.ptr
@@ -511,20 +546,106 @@ def make_task(algo: int, g: "DeviceCSR", out: DeviceBuffer, source: int = 0,
     t.out = out.ptr
     t.delta = delta
     t.hops = hops
+    if transpose is not None:
+        if transpose.n != g.n or transpose.m != g.m:
+            raise ValueError("transpose must have the graph's n and m")
+        t.trow, t.tcol = transpose.row.ptr, transpose.col.ptr
+        t.tw = transpose.w.ptr if transpose.w is not None else None
     t.rounds = 0
     return t
 
 
-def batch_solve(ctx: Context, tasks: list, team_groups: int = 1, opts=None):
-    """Runs the tasks in one persistent launch; returns (rounds per task, Stats)."""
+def make_selector(mode: int = SEL_NONE, fixed_arm: int = 0, narrow_groups: int = 1, seed: int = 0,
+                  weights: "DeviceBuffer | None" = None, trace: "DeviceBuffer | None" = None,
+                  trace_count: "DeviceBuffer | None" = None) -> Selector:
+    s = Selector()
+    s.mode, s.fixed_arm, s.narrow_groups, s.seed = mode, fixed_arm, narrow_groups, seed & 0xFFFFFFFF
+    s.weights = weights.ptr if weights is not None else None
+    if trace is not None:
+        if trace_count is None:
+            raise ValueError("a trace buffer needs a trace_count buffer")
+        s.trace = trace.ptr
+        s.trace_capacity = trace.count // TRACE_STRIDE
+        s.trace_count = trace_count.ptr
+    return s
+
+
+def batch_solve(ctx: Context, tasks: list, team_groups: int = 1, opts=None, selector: Selector | None = None):
+    """Runs the tasks in one persistent launch; returns (rounds per task, Stats).
+    With a selector, BFS / SSSP iterations pick their schedule arm per
+    iteration; each task's arm_iters counts the iterations per arm."""
     arr = (Task * len(tasks))(*tasks)
     st = Stats()
     o = _opts(opts)
-    check(lib().gdb_batch_solve(ctx.h, arr, len(tasks), team_groups, C.byref(o), C.byref(st)),
-          "gdb_batch_solve")
+    if selector is None:
+        check(lib().gdb_batch_solve(ctx.h, arr, len(tasks), team_groups, C.byref(o), C.byref(st)),
+              "gdb_batch_solve")
+    else:
+        check(lib().gdb_batch_solve_sel(ctx.h, arr, len(tasks), team_groups, C.byref(o), C.byref(selector),
+                                        C.byref(st)), "gdb_batch_solve_sel")
     for i, t in enumerate(tasks):
         t.rounds, t.time_us = arr[i].rounds, arr[i].time_us
+        t.arm_iters[:] = arr[i].arm_iters[:]
     return [arr[i].rounds for i in range(len(tasks))], st
+
+
+# ---------------------------------------------------------------------------
+# Selector MLP (forward / backward / Adam on the device)
+# ---------------------------------------------------------------------------
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """fp32 -> bf16 bit patterns (uint16), round to nearest even."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    nan = np.isnan(np.asarray(x, dtype=np.float32))
+    r[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
+    return r
+
+
+def bf16_to_f32(b: np.ndarray) -> np.ndarray:
+    return (np.asarray(b, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def mlp_init(seed: int = 5) -> np.ndarray:
+    """He-uniform initial parameters (fp32) in the library's layout."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((MLP_IN, MLP_HIDDEN), (MLP_HIDDEN, MLP_HIDDEN), (MLP_HIDDEN, MLP_OUT)):
+        lim = np.sqrt(6.0 / fan_in)
+        parts.append(rng.uniform(-lim, lim, size=fan_in * fan_out))
+        parts.append(np.zeros(fan_out))
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == MLP_PARAMS
+    return p
+
+
+def mlp_forward(ctx: Context, w_bf16: DeviceBuffer, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+    dx = DeviceBuffer.from_numpy(x)
+    dy = DeviceBuffer.empty(x.shape[0] * MLP_OUT, np.float32)
+    check(lib().gdb_mlp_forward(ctx.h, w_bf16.ptr, dx.ptr, x.shape[0], dy.ptr), "gdb_mlp_forward")
+    return dy.to_numpy().reshape(-1, MLP_OUT)
+
+
+def mlp_grad(ctx: Context, w_bf16: DeviceBuffer, records: DeviceBuffer, count: int):
+    """(gradient[MLP_PARAMS], loss, records used) of the trace loss."""
+    g = DeviceBuffer.empty(MLP_PARAMS, np.float32)
+    loss, used = C.c_float(0), C.c_int(0)
+    check(lib().gdb_mlp_grad(ctx.h, w_bf16.ptr, records.ptr, count, g.ptr, C.byref(loss), C.byref(used)),
+          "gdb_mlp_grad")
+    return g.to_numpy(), loss.value, used.value
+
+
+def mlp_train(ctx: Context, params: np.ndarray, records: DeviceBuffer, count: int, steps: int = 2000,
+              batch: int = 256, lr: float = 2e-3, seed: int = 5):
+    """Adam on the device; returns (fp32 params, bf16 weights as DeviceBuffer, final loss)."""
+    dp = DeviceBuffer.from_numpy(np.asarray(params, dtype=np.float32))
+    dm = DeviceBuffer.from_numpy(np.zeros(MLP_PARAMS, np.float32))
+    dv = DeviceBuffer.from_numpy(np.zeros(MLP_PARAMS, np.float32))
+    dw = DeviceBuffer.empty(MLP_PARAMS, np.uint16)
+    loss = C.c_float(0)
+    check(lib().gdb_mlp_train(ctx.h, dp.ptr, dm.ptr, dv.ptr, dw.ptr, records.ptr, count, steps, batch,
+                              C.c_float(lr), seed & 0xFFFFFFFF, C.byref(loss)), "gdb_mlp_train")
+    return dp.to_numpy(), dw, loss.value
 
 
 # ---------------------------------------------------------------------------

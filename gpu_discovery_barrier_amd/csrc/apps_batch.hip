@@ -17,8 +17,15 @@
 //          pile that is re-split when the near queue runs dry, threshold +=
 //          delta.  Same fixpoint (exact distances), far less re-relaxation.
 //   COLOR  Pannotia max rule, one barrier per round (see apps_color.hip).
+//
+// BFS / SSSP iterations may also run a schedule other than "push on the whole
+// team" (pull over in-edges, or on a narrow subset of the team's groups): the
+// group that completes each team barrier picks the next iteration's arm
+// (selector.hpp) and publishes it with the release.  All arms reach the same
+// fixpoint, so outputs do not depend on the choices.
 #include "graph_common.hpp"
 #include "runtime.hpp"
+#include "selector.hpp"
 
 namespace {
 
@@ -27,9 +34,19 @@ constexpr int kSerialLimit = 16;
 constexpr int kChunk = 4;                  // edges a lane relaxes per step
 constexpr int kLine = GDB_LINE_BYTES / 4;  // ints per 128 B line
 
-// Team state, one 128 B line per word.
+// Team state, one 128 B line per word.  TW_RESUME, TW_SEL and TW_CUM are
+// lines of several words (see below).
 enum TeamWord { TW_COUNT = 0, TW_GEN, TW_TASK, TW_TAIL0, TW_TAIL1, TW_TAIL2, TW_FAR0, TW_FAR1, TW_FAR2,
-                TW_STOP0, TW_STOP1, TW_STOP2, TW_WORDS };
+                TW_STOP0, TW_STOP1, TW_STOP2, TW_PARK, TW_RESUME, TW_SEL, TW_CUM, TW_WORDS };
+// TW_RESUME line: what a parked group needs to rejoin (written before the
+// release of TW_PARK).
+enum ResumeWord { RS_DONE = 0, RS_IT, RS_PAYLOAD, RS_FGEN, RS_THRESH, RS_BARGEN, RS_WORDS };
+// TW_SEL line (ints): [0,1] cumulative frontier edges at the last release,
+// [2,3] time of the last release, [4] trace record awaiting its cost.
+// TW_CUM line (u64): [0] out-edges of vertices pushed to the team queue,
+// [1] vertices reached, [2] out-edges of vertices reached.
+constexpr int kNinBits = 30;  // barrier payload = frontier size | arm << 30
+constexpr int kNinMask = (1 << kNinBits) - 1;
 
 struct DevTask {
   int algo, n, m, source, delta, hops;
@@ -39,6 +56,25 @@ struct DevTask {
   const float* nv;
   void* out;
   int* scratch;
+  const int* trow;
+  const int* tcol;
+  const int* tw;
+};
+
+// Selector inputs of one task, identical in every group of the team.
+struct SelRun {
+  gdbsel::Cfg cfg;
+  const unsigned short* lw;  // LDS copy of the weights (MLP mode) or nullptr
+  int task;
+  int Gn;          // groups of a narrow iteration
+  unsigned valid;  // arms this task can run
+};
+
+// Loop state a releasing group publishes to parked groups.
+struct IterState {
+  int it, arm, nin, fgen;
+  unsigned thresh;
+  int wakes;
 };
 
 constexpr int kLocalQ = 2048;  // LDS queue capacity per buffer (ints)
@@ -109,28 +145,183 @@ __device__ inline int team_barrier_payload(Team& T, const int* src, int* lds_val
   return *lds_val;
 }
 
-template <bool WEIGHTED>
-__device__ __forceinline__ unsigned relax_target(const int* __restrict__ col, const int* __restrict__ w,
-                                                 unsigned* dist, int k, unsigned dv, int n,
-                                                 int* dst_out) {
-  // Returns the new distance written to dst (kInf when nothing improved).
-  const int dst = col[k];
-  *dst_out = dst;
-  if ((unsigned)dst >= (unsigned)n) return kInf;
-  if (!WEIGHTED) {
-    if (dist[dst] != kInf) return kInf;
-    return atomicCAS(&dist[dst], kInf, dv) == kInf ? dv : kInf;
+__device__ __forceinline__ unsigned long long ld64(const void* p) {
+  return __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st64(void* p, unsigned long long v) {
+  __hip_atomic_store((unsigned long long*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Run by wave 0 of the group that completes a team barrier: fills in the
+// cost of the iteration that just ended in its trace record, computes the
+// next iteration's features and arm, appends its trace record, and returns
+// the payload (frontier size | arm << 30).
+__device__ int sel_release(Team& T, const int* tail, const SelRun& R, const DevTask& k,
+                           const IterState& S) {
+  const int lane = (int)__lane_id();
+  const int* sl = T.word(TW_SEL);
+  const unsigned long long* cum = (const unsigned long long*)T.word(TW_CUM);
+  // One load per lane, issued together.
+  unsigned long long x = 0;
+  if (lane == 0) x = ld64(sl);
+  else if (lane == 1) x = ld64(sl + 2);
+  else if (lane == 2) x = (unsigned)gdb::ld_rlx(sl + 4);
+  else if (lane == 3) x = ld64(cum);
+  else if (lane == 4) x = ld64(cum + 1);
+  else if (lane == 5) x = ld64(cum + 2);
+  else if (lane == 6) x = (unsigned)gdb::ld_rlx(tail);
+  auto get = [&](int l) -> unsigned long long {
+    const unsigned lo = (unsigned)__shfl((int)(unsigned)x, l, 64);
+    const unsigned hi = (unsigned)__shfl((int)(unsigned)(x >> 32), l, 64);
+    return ((unsigned long long)hi << 32) | lo;
+  };
+  const unsigned long long prev_fe = get(0), t_rel = get(1), cum_fe = get(3), cum_rc = get(4),
+                           cum_re = get(5);
+  const int pending = (int)(unsigned)get(2);
+  const int nin = (int)(unsigned)get(6);
+  const unsigned long long t = gdb::now();
+  const gdbsel::Cfg& c = R.cfg;
+  if (c.trace && lane == 0 && pending >= 0 && pending < c.trace_capacity)
+    c.trace[(size_t)pending * gdbsel::kStride + gdbsel::kIn + 1] = (float)(t - t_rel) * 0.01f;
+  int arm, rec = -1;
+  if (nin == 0) {
+    // A far-pile split (or the end): keep the team's width, nothing to learn.
+    arm = gdbsel::arm_narrow(S.arm) ? GDB_ARM_PUSH_NARROW : GDB_ARM_PUSH;
+    if (!(R.valid & (1u << arm))) arm = GDB_ARM_PUSH;
+  } else {
+    gdbsel::FeatIn a;
+    a.n = (float)k.n;
+    a.m = (float)k.m;
+    a.nin = (float)nin;
+    a.fe = (float)(cum_fe - prev_fe);
+    a.unv = (float)((long long)k.n - (long long)cum_rc);
+    a.unv_e = (float)((long long)k.m - (long long)cum_re);
+    a.prev_nin = (float)S.nin;
+    a.algo = k.algo;
+    a.prev_arm = S.arm;
+    a.team_threads = (float)(T.G * (int)blockDim.x);
+    a.narrow_threads = (float)(R.Gn * (int)blockDim.x);
+    const float xi = lane < gdbsel::kIn ? gdbsel::feature(a, lane) : 0.f;
+    arm = gdbsel::decide(c, R.lw, a, R.valid, (unsigned)R.task * 0x10001u + (unsigned)S.it, xi);
+    if (c.trace && c.trace_count) {
+      if (lane == 0) rec = atomicAdd(c.trace_count, 1);
+      rec = __shfl(rec, 0, 64);
+      if (rec < c.trace_capacity) {
+        float* r = c.trace + (size_t)rec * gdbsel::kStride;
+        if (lane < gdbsel::kIn) r[lane] = xi;
+        if (lane == 0) {
+          r[gdbsel::kIn] = (float)arm;
+          r[gdbsel::kIn + 1] = -1.f;
+          r[gdbsel::kIn + 2] = (float)R.task;
+          r[gdbsel::kIn + 3] = (float)(S.it + 1);
+        }
+      } else {
+        rec = -1;
+      }
+    }
   }
-  const unsigned wt = (unsigned)w[k];
-  if (wt >= kInf) return kInf;
-  const unsigned alt = dv + wt;
-  if (alt >= dist[dst]) return kInf;
-  const unsigned old = atomicMin(&dist[dst], alt);
-  return alt < old ? alt : kInf;
+  if (lane == 0) {
+    int* slw = T.word(TW_SEL);
+    st64(slw, cum_fe);
+    st64(slw + 2, t);
+    gdb::st_rlx(slw + 4, rec);
+  }
+  return nin | (arm << kNinBits);
+}
+
+// Team barrier over the `members` active groups that also picks the next
+// iteration's arm (in the group that completes it) and, when a narrow team
+// widens again, wakes the parked groups.  Returns the payload.
+__device__ int team_barrier_sel(Team& T, int members, const int* tail, const SelRun& R,
+                                const DevTask& k, const IterState& S) {
+  __shared__ int s_last, s_val, s_gen;
+  gdb::drain_stores();
+  __syncthreads();
+  const bool cross = members > 1;
+  unsigned long long* word = (unsigned long long*)T.word(TW_GEN);
+  if (threadIdx.x == 0) {
+    int g = *T.gen_cache;
+    if (g == GDB_GEN_UNKNOWN && T.G > 1)
+      g = (int)(unsigned)__hip_atomic_load(word, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    int last = 1;
+    if (cross) {
+      const int old = gdb::fadd(T.word(TW_COUNT), 1);
+      last = old == members - 1;
+      if (last) gdb::st_rlx(T.word(TW_COUNT), 0);
+    }
+    s_gen = g;
+    s_last = last;
+  }
+  __syncthreads();
+  if (s_last) {
+    if (threadIdx.x < 64) {
+      const int v = sel_release(T, tail, R, k, S);
+      if (threadIdx.x == 0) s_val = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int val = s_val;
+      const int new_arm = (int)((unsigned)val >> kNinBits);
+      const bool wake = members < T.G && !gdbsel::arm_narrow(new_arm);
+      const int ng = cross ? s_gen + 1 : s_gen;
+      if (wake) {
+        int* rs = T.word(TW_RESUME);
+        gdb::st_rlx(rs + RS_DONE, 0);
+        gdb::st_rlx(rs + RS_IT, S.it + 1);
+        gdb::st_rlx(rs + RS_PAYLOAD, val);
+        gdb::st_rlx(rs + RS_FGEN, S.fgen);
+        gdb::st_rlx(rs + RS_THRESH, (int)S.thresh);
+        gdb::st_rlx(rs + RS_BARGEN, ng);
+      }
+      if (cross) {
+        const unsigned long long w = ((unsigned long long)(unsigned)val << 32) | (unsigned)ng;
+        __hip_atomic_store(word, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (wake) gdb::st_rel(T.word(TW_PARK), S.wakes + 1);
+      *T.gen_cache = ng;
+    }
+  } else if (threadIdx.x == 0) {
+    const int g = s_gen;
+    unsigned long long w = 0;
+    const bool ok = gdb::wait_while(
+        T.ctx,
+        [&] {
+          w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          return (int)(unsigned)w == g;
+        },
+        GDB_ERR_BARRIER_TIMEOUT);
+    gdb::fence_acq();
+    if (!ok) T.l->aborted = 1;
+    s_val = (int)(w >> 32);
+    *T.gen_cache = g + 1;
+  }
+  __syncthreads();
+  return s_val;
+}
+
+// A parked group waits until the team's wake count passes `wakes`, then
+// reads the resume line.  Returns false when the wait gave up.
+__device__ bool team_park(Team& T, int wakes, int (&rs)[RS_WORDS]) {
+  __shared__ int s_rs[RS_WORDS];
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int* pw = T.word(TW_PARK);
+    const bool ok = gdb::wait_while(T.ctx, [&] { return gdb::ld_rlx(pw) <= wakes; },
+                                    GDB_ERR_BARRIER_TIMEOUT);
+    gdb::fence_acq();
+    s_ok = ok;
+    if (!ok) T.l->aborted = 1;
+    const int* r = T.word(TW_RESUME);
+    for (int i = 0; i < RS_WORDS; ++i) s_rs[i] = gdb::ld_rlx(r + i);
+    *T.gen_cache = s_rs[RS_BARGEN];
+  }
+  __syncthreads();
+  for (int i = 0; i < RS_WORDS; ++i) rs[i] = s_rs[i];
+  return s_ok != 0;
 }
 
 // ---- BFS / SSSP on a team --------------------------------------------------
-// scratch: stamp[n], fstamp[n], q0[n], q1[n], f0[n], f1[n], f2[n]
 // Adds a per-thread count into *dst with one atomic per wave.
 __device__ __forceinline__ void wave_add(unsigned long long* dst, long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -148,8 +339,9 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, long long v) {
 // local steps (workgroup barriers only), so one team barrier covers several
 // hops; what is left spills to the team's queue.  Any relaxation order
 // reaches the same fixpoint (exact levels / distances).
-template <bool WEIGHTED, bool UNIT>
-__device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work) {
+template <bool WEIGHTED, bool UNIT, bool SEL>
+__device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work, const SelRun& R,
+                             int (&arm_iters)[4]) {
   // scratch: BFS (CAS)  q0[n], q1[n]
   //          otherwise  d64[n] (8 B each), fstamp[n], gstamp[n], q0[n], q1[n], far0..2[n]
   // Marks in the low word of d64 (level-synchronous iterations only):
@@ -162,6 +354,7 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
   int (*lq)[kLocalQ] = T.lq;
   int* lcnt = T.lcnt;
   const int n = k.n;
+  const int G = T.G;
   unsigned* dist = (unsigned*)k.out;
   unsigned long long* d64 = (unsigned long long*)k.scratch;
   int* base = WEIGHTED ? k.scratch + 2 * (size_t)n : k.scratch;
@@ -169,55 +362,176 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
   int* gstamp = base + n;
   int* q[2] = {WEIGHTED ? base + 2 * (size_t)n : base, WEIGHTED ? base + 3 * (size_t)n : base + n};
   int* far[3] = {base + 4 * (size_t)n, base + 5 * (size_t)n, base + 6 * (size_t)n};
-  const int tid = T.tid(), stride = T.stride();
   const bool near_far = WEIGHTED && !UNIT && k.delta > 0;
   const int hops = WEIGHTED ? k.hops : 0;
-  for (int v = tid; v < n; v += stride) {
-    if (WEIGHTED) {
-      d64[v] = (unsigned long long)(v == k.source ? 0u : kInf) << 32;
-      fstamp[v] = -1;
-      gstamp[v] = 0;
-    } else {
-      dist[v] = v == k.source ? 0u : kInf;
+  // The selector runs on level-synchronous tasks only (no local hops).  The
+  // SEL = false instantiation carries none of its code (fewer registers).
+  const bool sel = SEL && !UNIT && R.cfg.mode != GDB_SEL_NONE && k.hops == 0;
+  const int Gn = sel ? R.Gn : G;
+  {
+    const int tid = T.tid(), stride = T.stride();
+    for (int v = tid; v < n; v += stride) {
+      if (WEIGHTED) {
+        d64[v] = (unsigned long long)(v == k.source ? 0u : kInf) << 32;
+        fstamp[v] = -1;
+        gstamp[v] = 0;
+      } else {
+        dist[v] = v == k.source ? 0u : kInf;
+      }
+    }
+    if (tid == 0) {
+      q[0][0] = k.source;
+      gdb::st_rlx(T.word(TW_TAIL1), 1);
+      gdb::st_rlx(T.word(TW_TAIL2), 0);
+      gdb::st_rlx(T.word(TW_TAIL0), 0);
+      gdb::st_rlx(T.word(TW_FAR0), 0);
+      gdb::st_rlx(T.word(TW_FAR1), 0);
+      gdb::st_rlx(T.word(TW_FAR2), 0);
+      if (sel) {
+        const unsigned long long d0 = (unsigned long long)(k.row[k.source + 1] - k.row[k.source]);
+        int* sl = T.word(TW_SEL);
+        unsigned long long* cum = (unsigned long long*)T.word(TW_CUM);
+        st64(sl, 0);
+        st64(sl + 2, gdb::now());
+        gdb::st_rlx(sl + 4, -1);
+        st64(cum, d0);
+        st64(cum + 1, 1);
+        st64(cum + 2, d0);
+      }
     }
   }
-  if (tid == 0) {
-    q[0][0] = k.source;
-    gdb::st_rlx(T.word(TW_TAIL1), 1);
-    gdb::st_rlx(T.word(TW_TAIL2), 0);
-    gdb::st_rlx(T.word(TW_TAIL0), 0);
-    gdb::st_rlx(T.word(TW_FAR0), 0);
-    gdb::st_rlx(T.word(TW_FAR1), 0);
-    gdb::st_rlx(T.word(TW_FAR2), 0);
-  }
-  if (!team_barrier(T)) return -1;
   unsigned thresh = near_far ? (unsigned)k.delta : kInf;
   int gen = 0;  // far pile generation being filled
   int it = 1;
   int nin = 1;
+  int arm = GDB_ARM_PUSH;
+  int wakes = 0;  // wake-ups of parked groups seen so far
+  if (SEL && sel) {
+    const IterState S0{0, GDB_ARM_PUSH, 0, 0, thresh, 0};
+    const int pl = team_barrier_sel(T, G, T.word(TW_TAIL1), R, k, S0);
+    if (T.l->aborted) return -1;
+    nin = pl & kNinMask;
+    arm = (int)((unsigned)pl >> kNinBits);
+  } else if (!team_barrier(T)) {
+    return -1;
+  }
   long long items = 0, edges = 0;
+  long long f_fe = 0, f_rc = 0, f_re = 0;  // selector features (this thread's share)
   for (;;) {
+    const int width = gdbsel::arm_narrow(arm) ? Gn : G;
+    if (SEL && T.rank >= width) {
+      // Parked until the active groups widen the team again or finish.
+      int rs[RS_WORDS];
+      if (!team_park(T, wakes, rs)) return -1;
+      ++wakes;
+      it = rs[RS_IT];
+      if (rs[RS_DONE]) break;
+      nin = rs[RS_PAYLOAD] & kNinMask;
+      arm = (int)((unsigned)rs[RS_PAYLOAD] >> kNinBits);
+      gen = rs[RS_FGEN];
+      thresh = (unsigned)rs[RS_THRESH];
+      continue;
+    }
+    const int tid = T.rank * (int)blockDim.x + (int)threadIdx.x, stride = width * (int)blockDim.x;
+    if (T.rank == 0 && threadIdx.x == 0 && nin > 0) ++arm_iters[arm & 3];  // relaxing iterations
     const int* in = q[(it - 1) & 1];
     int* outq = q[it & 1];
     int* out_tail = T.word(TW_TAIL0 + (it + 1) % 3);
     if (tid == 0) gdb::st_rlx(T.word(TW_TAIL0 + (it + 2) % 3), 0);
     const unsigned near_mark = 2u * (unsigned)it, far_mark = near_mark + 1u;
-    if (nin > 0) {
+    int* fpile = far[gen % 3];
+    int* ftail = T.word(TW_FAR0 + gen % 3);
+    // Selector features of a vertex entering the next frontier (to_team) or
+    // reached for the first time.
+    auto account = [&](int v, bool to_team, bool first) {
+      if (!sel || !(to_team || first) || (unsigned)v >= (unsigned)n) return;
+      const long long d = k.row[v + 1] - k.row[v];
+      if (to_team) f_fe += d;
+      if (first) {
+        ++f_rc;
+        f_re += d;
+      }
+    };
+    if (SEL && !UNIT && nin > 0 && gdbsel::arm_pull(arm)) {
       if (nin > n) {
         gdb::set_error(T.ctx, GDB_ERR_QUEUE_OVERFLOW);
         return -1;
       }
-      int* fpile = far[gen % 3];
-      int* ftail = T.word(TW_FAR0 + gen % 3);
+      // Pull: every vertex scans its in-edges.  BFS: an unvisited vertex
+      // with an in-neighbour on level it-1 takes level it (bottom-up, first
+      // hit ends the scan).  SSSP: d[v] = min(d[v], min d[u] + w(u, v)).
+      // Each vertex is written by its own thread only.
+      if constexpr (SEL && !UNIT) {
+        GDB_FOR_UNIFORM(v, valid, tid, stride, n) {
+          int b = 0, e = 0;
+          bool cand = false;
+          unsigned cur = kInf;
+          if (valid) {
+            cur = WEIGHTED ? (unsigned)(d64[v] >> 32) : dist[v];
+            cand = WEIGHTED || cur == kInf;
+            if (cand) {
+              b = k.trow[v];
+              e = k.trow[v + 1];
+              ++items;
+              edges += e - b;
+            }
+          }
+          unsigned best = kInf;
+          if constexpr (!WEIGHTED) {
+            const unsigned lvl = (unsigned)(it - 1);
+            const bool found = gdb::wave_coop_any(
+                cand, b, e,
+                [&](int kk) {
+                  const int u = k.tcol[kk];
+                  return (unsigned)u < (unsigned)n && dist[u] == lvl;
+                },
+                kSerialLimit);
+            if (found) best = (unsigned)it;
+          } else {
+            best = gdb::wave_coop_reduce(
+                cand, b, e, kInf,
+                [&](int kk) -> unsigned {
+                  const int u = k.tcol[kk];
+                  if ((unsigned)u >= (unsigned)n) return kInf;
+                  const unsigned wt = (unsigned)k.tw[kk];
+                  const unsigned du = (unsigned)(d64[u] >> 32);
+                  return (du < kInf && wt < kInf) ? du + wt : kInf;
+                },
+                [](unsigned x, unsigned y) { return x < y ? x : y; }, kSerialLimit);
+          }
+          int cls = 0;
+          bool first = false;
+          if (valid && best < cur) {
+            first = cur == kInf;
+            if (!WEIGHTED) {
+              dist[v] = best;
+              cls = 1;
+            } else {
+              const bool nearp = best < thresh;
+              d64[v] = ((unsigned long long)best << 32) | (nearp ? near_mark : far_mark);
+              cls = nearp ? 1 : (atomicExch(&fstamp[v], gen) != gen ? 2 : 0);
+            }
+          }
+          account(v, cls == 1, first);
+          gdb::wave_push(cls == 1, v, outq, out_tail);
+          if (near_far) gdb::wave_push(cls == 2, v, fpile, ftail);
+        }
+      }
+    } else if (nin > 0) {
+      if (nin > n) {
+        gdb::set_error(T.ctx, GDB_ERR_QUEUE_OVERFLOW);
+        return -1;
+      }
       if (hops > 0 && threadIdx.x < 3) lcnt[threadIdx.x] = 0;
       if (hops > 0) __syncthreads();
       // Relaxes edge kk from a vertex at distance src_d.  Returns the push
-      // class: 0 none, 1 team queue, 2 far pile, 3 local (LDS) queue.
+      // class (0 none, 1 team queue, 2 far pile, 3 local LDS queue), plus 4
+      // when dst was reached for the first time.
       auto relax = [&](int kk, unsigned src_d, int* dst_out) -> int {
         const int dst = k.col[kk];
         *dst_out = dst;
         if ((unsigned)dst >= (unsigned)n) return 0;
-        if (!WEIGHTED) return atomicCAS(&dist[dst], kInf, src_d) == kInf ? 1 : 0;
+        if (!WEIGHTED) return atomicCAS(&dist[dst], kInf, src_d) == kInf ? 1 | 4 : 0;
         const unsigned wt = UNIT ? 1u : (unsigned)k.w[kk];
         if (wt >= kInf) return 0;
         const unsigned cand = src_d + wt;
@@ -230,9 +544,11 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
         }
         const unsigned long long nw = ((unsigned long long)cand << 32) | (nearp ? near_mark : far_mark);
         const unsigned long long old = atomicMin(&d64[dst], nw);
-        if (cand >= (unsigned)(old >> 32)) return 0;
-        if (nearp) return (unsigned)old == near_mark ? 0 : 1;
-        return atomicExch(&fstamp[dst], gen) != gen ? 2 : 0;
+        const unsigned oldd = (unsigned)(old >> 32);
+        if (cand >= oldd) return 0;
+        const int fr = oldd == kInf ? 4 : 0;
+        if (nearp) return ((unsigned)old == near_mark ? 0 : 1) | fr;
+        return (atomicExch(&fstamp[dst], gen) != gen ? 2 : 0) | fr;
       };
       // Pushes the results of one relaxation step of this wave.  Class 3
       // goes to LDS buffer `lb` when there is room, else to the team queue.
@@ -240,14 +556,16 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
         bool to_team[kChunk], to_far[kChunk];
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
-          bool team_push = cls[j] == 1;
-          if (cls[j] == 3) {
+          const int c = cls[j] & 3;
+          bool team_push = c == 1;
+          if (c == 3) {
             const int pos = atomicAdd(lc, 1);
             if (pos < kLocalQ) lq[lb][pos] = dst[j];
             else team_push = atomicExch(&gstamp[dst[j]], it) != it;
           }
           to_team[j] = team_push;
-          to_far[j] = cls[j] == 2;
+          to_far[j] = c == 2;
+          account(dst[j], team_push, (cls[j] & 4) != 0);
         }
         gdb::wave_push_n<kChunk>(to_team, dst, outq, out_tail);
         if (near_far) gdb::wave_push_n<kChunk>(to_far, dst, fpile, ftail);
@@ -324,7 +642,17 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
       // Near queue empty: split the far pile of generation `gen` at the next
       // threshold into the near queue and generation gen+1.
       const int fsz = gdb::ld_rlx(T.word(TW_FAR0 + gen % 3));
-      if (!near_far || fsz == 0) break;
+      if (!near_far || fsz == 0) {
+        if (width < G && T.rank == 0 && threadIdx.x == 0) {
+          // Finished on a narrow team: release the parked groups.
+          int* rs = T.word(TW_RESUME);
+          gdb::st_rlx(rs + RS_DONE, 1);
+          gdb::st_rlx(rs + RS_IT, it);
+          gdb::st_rlx(rs + RS_BARGEN, *T.gen_cache);
+          gdb::st_rel(T.word(TW_PARK), wakes + 1);
+        }
+        break;
+      }
       if (fsz > n) {
         gdb::set_error(T.ctx, GDB_ERR_QUEUE_OVERFLOW);
         return -1;
@@ -344,17 +672,33 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
             else to_far = atomicExch(&fstamp[v], gen + 1) != gen + 1;
           }
         }
+        account(v, to_near, false);
         gdb::wave_push(to_near, v, outq, out_tail);
         gdb::wave_push(to_far, v, next_pile, next_tail);
       }
       ++gen;
     }
-    nin = team_barrier_payload(T, out_tail, &lds_n);
-    if (T.l->aborted) return -1;
+    if (SEL && sel) {
+      unsigned long long* cum = (unsigned long long*)T.word(TW_CUM);
+      wave_add(cum, f_fe);
+      wave_add(cum + 1, f_rc);
+      wave_add(cum + 2, f_re);
+      f_fe = f_rc = f_re = 0;
+      const IterState S{it, arm, nin, gen, thresh, wakes};
+      const int pl = team_barrier_sel(T, width, out_tail, R, k, S);
+      if (T.l->aborted) return -1;
+      const int new_arm = (int)((unsigned)pl >> kNinBits);
+      if (width < G && !gdbsel::arm_narrow(new_arm)) ++wakes;  // the releaser woke the parked groups
+      nin = pl & kNinMask;
+      arm = new_arm;
+    } else {
+      nin = team_barrier_payload(T, out_tail, &lds_n);
+      if (T.l->aborted) return -1;
+    }
     ++it;
   }
   if (WEIGHTED)
-    for (int v = tid; v < n; v += stride) dist[v] = (unsigned)(d64[v] >> 32);
+    for (int v = T.tid(); v < n; v += T.stride()) dist[v] = (unsigned)(d64[v] >> 32);
   wave_add(&work[0], items);
   wave_add(&work[1], edges);
   return it - 1;
@@ -425,15 +769,16 @@ __device__ int team_color(Team& T, const DevTask& k) {
   return r;
 }
 
-template <int WGS>
+template <int WGS, bool SEL>
 __global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevTask* __restrict__ tasks,
                                                int ntasks, int G, int* team_state, int* next_task,
                                                int* results, int* teams_used,
-                                               unsigned long long* work) {
+                                               unsigned long long* work, gdbsel::Cfg sel) {
   DISCOVERY_PROTOCOL(ctx);
   __shared__ int gen_cache;
   __shared__ int lq[2][kLocalQ];
   __shared__ int lcnt[3];
+  extern __shared__ unsigned short sel_w[];  // MLP weights (dynamic LDS, MLP mode only)
   const int P = p_get_num_groups(ctx, &local_ctx);
   const int id = p_get_group_id(ctx, &local_ctx);
   if (G > P) G = P;
@@ -442,27 +787,41 @@ __global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevT
   const int team = id / G;
   if (team >= nteams) return;
   if (threadIdx.x == 0) gen_cache = GDB_GEN_UNKNOWN;
+  const bool mlp = SEL && sel.mode == GDB_SEL_MLP && sel.w;
+  if (mlp)
+    for (int i = threadIdx.x; i < gdbsel::kParams; i += blockDim.x) sel_w[i] = sel.w[i];
   Team T{ctx, &local_ctx, team_state + (size_t)team * TW_WORDS * kLine, &gen_cache, G, id % G, lq, lcnt};
   if (id == 0 && threadIdx.x == 0) *teams_used = nteams;
+  int Gn = sel.narrow_groups > 0 ? sel.narrow_groups : 1;
+  if (Gn > G) Gn = G;
   for (;;) {
     if (T.rank == 0 && threadIdx.x == 0) gdb::st_rlx(T.word(TW_TASK), gdb::fadd_rlx(next_task, 1));
     if (!team_barrier(T)) return;
     const int t = gdb::ld_rlx(T.word(TW_TASK));
     if (t >= ntasks) break;
     const DevTask k = tasks[t];
-    if (T.tid() == 0) work[4 * (size_t)t + 2] = gdb::now();
+    if (T.tid() == 0) work[8 * (size_t)t + 2] = gdb::now();
     int rounds = 0;
+    int arm_iters[4] = {0, 0, 0, 0};
     if (k.n > 0) {
-      unsigned long long* wk = work + 4 * (size_t)t;
+      unsigned long long* wk = work + 8 * (size_t)t;
+      const bool pull_ok = k.trow && k.tcol && (k.algo == GDB_TASK_BFS || k.tw);
+      unsigned valid = 1u << GDB_ARM_PUSH;
+      if (pull_ok) valid |= 1u << GDB_ARM_PULL;
+      if (Gn < G) valid |= 1u << GDB_ARM_PUSH_NARROW;
+      if (pull_ok && Gn < G) valid |= 1u << GDB_ARM_PULL_NARROW;
+      const SelRun R{sel, mlp ? sel_w : nullptr, t, Gn, valid};
       if (k.algo == GDB_TASK_BFS)
-        rounds = k.hops > 0 ? team_worklist<true, true>(T, k, wk) : team_worklist<false, false>(T, k, wk);
-      else if (k.algo == GDB_TASK_SSSP) rounds = team_worklist<true, false>(T, k, wk);
+        rounds = k.hops > 0 ? team_worklist<true, true, false>(T, k, wk, R, arm_iters)
+                            : team_worklist<false, false, SEL>(T, k, wk, R, arm_iters);
+      else if (k.algo == GDB_TASK_SSSP) rounds = team_worklist<true, false, SEL>(T, k, wk, R, arm_iters);
       else rounds = team_color(T, k);
     }
     if (rounds < 0) return;
     if (T.tid() == 0) {
       results[t] = rounds;
-      work[4 * (size_t)t + 3] = gdb::now();
+      work[8 * (size_t)t + 3] = gdb::now();
+      for (int a = 0; a < 4; ++a) work[8 * (size_t)t + 4 + a] = (unsigned long long)arm_iters[a];
     }
     if (!team_barrier(T)) return;
   }
@@ -472,12 +831,20 @@ __global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevT
 
 using namespace gdbrt;
 
-extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups,
-                               const gdb_opts* opts, gdb_stats* stats) {
+extern "C" int gdb_batch_solve_sel(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups,
+                                   const gdb_opts* opts, const gdb_selector* sel, gdb_stats* stats) {
   if (!ctx || ntasks < 0 || (ntasks > 0 && !tasks) || team_groups < 0) return GDB_E_INVALID;
   gdb_opts o{};
   if (opts) o = *opts;
   const int wgs = default_wgs(&o);
+  gdbsel::Cfg cfg{GDB_SEL_NONE, 0, 1, 0u, nullptr, nullptr, 0, nullptr};
+  if (sel) {
+    if (sel->mode < GDB_SEL_NONE || sel->mode > GDB_SEL_MLP || sel->narrow_groups < 0 ||
+        sel->trace_capacity < 0 || (sel->trace && !sel->trace_count))
+      return GDB_E_INVALID;
+    cfg = gdbsel::Cfg{sel->mode, sel->fixed_arm, sel->narrow_groups, sel->seed, sel->weights,
+                      sel->trace, sel->trace ? sel->trace_capacity : 0, sel->trace_count};
+  }
   gdb_stats st{};
   if (ntasks == 0) {
     if (stats) *stats = st;
@@ -486,30 +853,31 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
   // Per-task scratch sizes (ints) and the device task table.
   std::vector<DevTask> dt((size_t)ntasks);
   size_t scratch_total = 0;
-  long long max_n = 0;
   for (int t = 0; t < ntasks; ++t) {
     const gdb_task& a = tasks[t];
     if (a.n < 0 || a.m < 0 || a.algo < GDB_TASK_BFS || a.algo > GDB_TASK_COLOR || !a.out ||
         (a.n > 0 && !a.row) || (a.m > 0 && !a.col) || (a.algo == GDB_TASK_SSSP && a.m > 0 && !a.w) ||
         (a.algo == GDB_TASK_COLOR && a.n > 0 && !a.node_value) ||
         (a.algo != GDB_TASK_COLOR && a.n > 0 && (a.source < 0 || a.source >= a.n)) || a.delta < 0 ||
-        a.hops < 0 || a.hops > 64)
+        a.hops < 0 || a.hops > 64 || (a.trow && !a.tcol) ||
+        (cfg.mode != GDB_SEL_NONE && a.algo != GDB_TASK_COLOR && a.n > kNinMask))
       return GDB_E_INVALID;
     const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : 2) * a.n;
     dt[t] = DevTask{a.algo, a.n, a.m, a.source, a.delta, a.hops, a.row, a.col, a.w, a.node_value, a.out,
-                    nullptr};
+                    nullptr, a.trow, a.tcol, a.tw};
     scratch_total += (ints + 63) & ~(size_t)63;
-    if (a.n > max_n) max_n = a.n;
   }
+  const size_t dyn_lds = (cfg.mode == GDB_SEL_MLP && cfg.w) ? sizeof(unsigned short) * gdbsel::kParams : 0;
   int groups = 0;
   int rc = gdb_dispatch_wgs(wgs, [&](auto W) -> int {
     constexpr int WG = decltype(W)::value;
-    return launch_groups(ctx, (const void*)k_batch<WG>, &o, (long long)GDB_MAX_GROUPS * WG, &groups);
+    const void* kern = cfg.mode != GDB_SEL_NONE ? (const void*)k_batch<WG, true> : (const void*)k_batch<WG, false>;
+    return launch_groups(ctx, kern, &o, (long long)GDB_MAX_GROUPS * WG, &groups, dyn_lds);
   });
   if (rc) return rc;
   const size_t team_ints = (size_t)groups * TW_WORDS * kLine;
   const size_t bytes = carve_bytes<DevTask>(ntasks) + carve_bytes<int>(team_ints) +
-                       carve_bytes<int>(ntasks + 64) + carve_bytes<unsigned long long>(4 * (size_t)ntasks) +
+                       carve_bytes<int>(ntasks + 64) + carve_bytes<unsigned long long>(8 * (size_t)ntasks) +
                        carve_bytes<int>(scratch_total);
   rc = ensure_workspace(ctx, bytes);
   if (rc) return rc;
@@ -517,7 +885,7 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
   DevTask* d_tasks = cv.take<DevTask>(ntasks);
   int* team_state = cv.take<int>(team_ints);
   int* misc = cv.take<int>(ntasks + 64);
-  unsigned long long* work = cv.take<unsigned long long>(4 * (size_t)ntasks);
+  unsigned long long* work = cv.take<unsigned long long>(8 * (size_t)ntasks);
   int* scratch = cv.take<int>(scratch_total);
   size_t off = 0;
   for (int t = 0; t < ntasks; ++t) {
@@ -528,7 +896,7 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
   GDB_HIP(hipMemcpyAsync(d_tasks, dt.data(), sizeof(DevTask) * ntasks, hipMemcpyHostToDevice, ctx->stream));
   rc = fill_i32(ctx, team_state, 0, (long long)team_ints);
   if (!rc) rc = fill_i32(ctx, misc, 0, ntasks + 64);
-  if (!rc) rc = fill_i32(ctx, (int*)work, 0, 8 * (long long)ntasks);
+  if (!rc) rc = fill_i32(ctx, (int*)work, 0, 16 * (long long)ntasks);
   if (rc) return rc;
   int* next_task = misc;
   int* teams_used = misc + 32;
@@ -540,26 +908,32 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
     constexpr int WG = decltype(W)::value;
     int r2 = timer_start(ctx);
     if (r2) return r2;
-    hipLaunchKernelGGL(k_batch<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, d_tasks, ntasks, G,
-                       team_state, next_task, results, teams_used, work);
+    if (cfg.mode != GDB_SEL_NONE)
+      hipLaunchKernelGGL((k_batch<WG, true>), dim3(groups), dim3(WG), dyn_lds, ctx->stream, ctx->d, d_tasks,
+                         ntasks, G, team_state, next_task, results, teams_used, work, cfg);
+    else
+      hipLaunchKernelGGL((k_batch<WG, false>), dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, d_tasks, ntasks,
+                         G, team_state, next_task, results, teams_used, work, cfg);
     GDB_HIP(hipGetLastError());
     return timer_stop(ctx, &st.kernel_ms);
   });
   if (rc) return rc;
   std::vector<int> hres((size_t)ntasks);
-  std::vector<unsigned long long> hwork((size_t)4 * ntasks);
+  std::vector<unsigned long long> hwork((size_t)8 * ntasks);
   GDB_HIP(hipMemcpy(hres.data(), results, sizeof(int) * ntasks, hipMemcpyDeviceToHost));
-  GDB_HIP(hipMemcpy(hwork.data(), work, sizeof(unsigned long long) * 4 * ntasks, hipMemcpyDeviceToHost));
+  GDB_HIP(hipMemcpy(hwork.data(), work, sizeof(unsigned long long) * 8 * ntasks, hipMemcpyDeviceToHost));
   for (int t = 0; t < ntasks; ++t) {
+    const unsigned long long* wk = &hwork[(size_t)8 * t];
     tasks[t].rounds = hres[t];
     st.rounds += hres[t];
     // Worklist tasks report items / edges processed; colour tasks count the
     // graph once.
     const bool wl = tasks[t].algo != GDB_TASK_COLOR;
-    st.work_items += wl ? (long long)hwork[4 * t] : tasks[t].n;
-    st.work_edges += wl ? (long long)hwork[4 * t + 1] : tasks[t].m;
+    st.work_items += wl ? (long long)wk[0] : tasks[t].n;
+    st.work_edges += wl ? (long long)wk[1] : tasks[t].m;
     // Device wall clock (100 MHz) from the team's start to end of the task.
-    tasks[t].time_us = (double)(hwork[4 * t + 3] - hwork[4 * t + 2]) / 100.0;
+    tasks[t].time_us = (double)(wk[3] - wk[2]) / 100.0;
+    for (int a = 0; a < 4; ++a) tasks[t].arm_iters[a] = (int)wk[4 + a];
   }
   st.launches = 1;
   rc = read_participating(ctx, &st.participating);
@@ -567,4 +941,9 @@ extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int te
   rc = check_device_error(ctx);
   if (stats) *stats = st;
   return rc;
+}
+
+extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups,
+                               const gdb_opts* opts, gdb_stats* stats) {
+  return gdb_batch_solve_sel(ctx, tasks, ntasks, team_groups, opts, nullptr, stats);
 }

@@ -9,6 +9,9 @@ __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
 __device__ __forceinline__ float shfl_xor(float x, int o) { return __shfl_xor(x, o, 64); }
 __device__ __forceinline__ int shfl_xor(int x, int o) { return __shfl_xor(x, o, 64); }
+__device__ __forceinline__ unsigned shfl_xor(unsigned x, int o) {
+  return (unsigned)__shfl_xor((int)x, o, 64);
+}
 
 struct MaxFound {  // running max plus "at least one candidate seen"
   float mx;
@@ -77,6 +80,40 @@ __device__ __forceinline__ void wave_coop_foreach(bool active, int b, int e, uns
     for (int base = lb; base < le; base += 64) fn(base + lane, base + lane < le, lp);
     mask &= mask - 1;
   }
+}
+
+// Wave cooperative "some edge k in [b, e) has pred(k)" with early exit:
+// short ranges are scanned by their own lane up to the first hit, long ones
+// by the whole wave, 64 consecutive edges per step, up to the first step
+// with a hit.  All 64 lanes must call it.
+template <typename P>
+__device__ __forceinline__ bool wave_coop_any(bool active, int b, int e, P pred, int serial_limit) {
+  const bool heavy = active && (e - b) > serial_limit;
+  bool found = false;
+  if (active && !heavy)
+    for (int k = b; k < e; ++k)
+      if (pred(k)) {
+        found = true;
+        break;
+      }
+  unsigned long long mask = __ballot(heavy);
+  const int lane = lane_id();
+  while (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    const int lb = __shfl(b, leader, 64);
+    const int le = __shfl(e, leader, 64);
+    bool hit = false;
+    for (int base = lb; base < le; base += 64) {
+      const bool h = base + lane < le && pred(base + lane);
+      if (__ballot(h) != 0ull) {
+        hit = true;
+        break;
+      }
+    }
+    if (lane == leader) found = hit;
+    mask &= mask - 1;
+  }
+  return found;
 }
 
 // Appends `item` for every lane with push == true to queue[*tail ...]:

@@ -146,11 +146,11 @@ int occupancy_bound(gdb_ctx* ctx, const void* kernel, int wgs, size_t lds, int* 
 }
 
 int launch_groups(gdb_ctx* ctx, const void* kernel, const gdb_opts* opts, long long work_items,
-                  int* out) {
+                  int* out, size_t dyn_lds) {
   const int wgs = default_wgs(opts);
   int g = (opts && opts->groups > 0) ? opts->groups : 0;
   if (g == 0) {
-    int rc = occupancy_bound(ctx, kernel, wgs, 0, &g);
+    int rc = occupancy_bound(ctx, kernel, wgs, dyn_lds, &g);
     if (rc) return rc;
   }
   long long need = (work_items + wgs - 1) / wgs;

@@ -72,9 +72,9 @@ int check_device_error(gdb_ctx* ctx);
 int occupancy_bound(gdb_ctx* ctx, const void* kernel, int wgs, size_t lds, int* out);
 
 // Resolves the number of groups to launch: opts->groups if positive, else the
-// occupancy bound, clamped to [1, min(work_groups, GDB_MAX_GROUPS)].
+// occupancy bound (with `dyn_lds` bytes of dynamic LDS per group), clamped to [1, min(work_groups, GDB_MAX_GROUPS)].
 int launch_groups(gdb_ctx* ctx, const void* kernel, const gdb_opts* opts, long long work_items,
-                  int* out);
+                  int* out, size_t dyn_lds = 0);
 
 inline int default_wgs(const gdb_opts* o) { return (o && o->wgs > 0) ? o->wgs : 256; }
 

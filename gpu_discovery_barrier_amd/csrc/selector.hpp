@@ -1,0 +1,157 @@
+// Per-iteration schedule selector of the batched BFS / SSSP engine: the
+// iteration features, the 16-64-64-4 MLP evaluated by one wave (bf16
+// weights, fp32 accumulation), the plain heuristic it falls back to, the
+// random arm used to collect traces, and the trace records.
+//
+// One wave of the group that completes a team barrier runs it; inputs are
+// uniform across that wave, so every lane returns the same arm.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/gdb_amd.h"
+
+namespace gdbsel {
+
+constexpr int kIn = GDB_MLP_IN, kHid = GDB_MLP_HIDDEN, kOut = GDB_MLP_OUT;
+// Input-major parameter layout: layer output j = b[j] + sum_i W[i][j] x[i],
+// so lane j of a wave reads W[i][j] at consecutive addresses.
+constexpr int kW1 = 0, kB1 = kW1 + kIn * kHid, kW2 = kB1 + kHid, kB2 = kW2 + kHid * kHid,
+              kW3 = kB2 + kHid, kB3 = kW3 + kHid * kOut, kParams = kB3 + kOut;
+static_assert(kParams == GDB_MLP_PARAMS, "parameter count");
+constexpr int kStride = GDB_TRACE_STRIDE;
+
+struct Cfg {
+  int mode, fixed_arm, narrow_groups;
+  unsigned seed;
+  const unsigned short* w;
+  float* trace;
+  int trace_capacity;
+  int* trace_count;
+};
+
+__device__ __forceinline__ float bf16(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// Round to nearest even (NaN stays NaN).
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ bool arm_pull(int a) { return a == GDB_ARM_PULL || a == GDB_ARM_PULL_NARROW; }
+__device__ __forceinline__ bool arm_narrow(int a) { return a >= GDB_ARM_PUSH_NARROW; }
+
+// Iteration features (all lanes compute the same values).
+struct FeatIn {
+  float n, m;           // graph size
+  float nin, fe;        // next frontier: vertices, out-edges
+  float unv, unv_e;     // vertices not reached yet, and their out-edges
+  float prev_nin;       // frontier of the iteration that just ended
+  int algo, prev_arm;
+  float team_threads, narrow_threads;
+};
+
+__device__ __forceinline__ float lg(float x) { return __log2f(1.f + fmaxf(x, 0.f)) * (1.f / 32.f); }
+
+__device__ __forceinline__ float feature(const FeatIn& a, int i) {
+  switch (i) {
+    case 0: return lg(a.nin);
+    case 1: return lg(a.fe);
+    case 2: return lg(a.unv);
+    case 3: return lg(a.unv_e);
+    case 4: return lg(a.n);
+    case 5: return lg(a.m);
+    case 6: return a.nin / fmaxf(a.n, 1.f);
+    case 7: return a.fe / fmaxf(a.m, 1.f);
+    case 8: return a.unv / fmaxf(a.n, 1.f);
+    case 9: return a.unv_e / fmaxf(a.m, 1.f);
+    case 10: return (float)a.algo;
+    case 11: return arm_pull(a.prev_arm) ? 1.f : 0.f;
+    case 12: return arm_narrow(a.prev_arm) ? 1.f : 0.f;
+    case 13: return lg(a.prev_nin);
+    case 14: return lg(a.team_threads);
+    default: return lg(a.narrow_threads);
+  }
+}
+
+// y = MLP(x) for the features held by lanes 0..15 (`xi`); every lane of the
+// wave must call it and gets all four outputs.
+__device__ inline void mlp_wave(const unsigned short* __restrict__ w, float xi, float (&y)[kOut]) {
+  const int lane = (int)__lane_id();
+  float h1 = bf16(w[kB1 + lane]);
+#pragma unroll
+  for (int i = 0; i < kIn; ++i) h1 += bf16(w[kW1 + i * kHid + lane]) * __shfl(xi, i, 64);
+  h1 = fmaxf(h1, 0.f);
+  float h2 = bf16(w[kB2 + lane]);
+#pragma unroll 16
+  for (int i = 0; i < kHid; ++i) h2 += bf16(w[kW2 + i * kHid + lane]) * __shfl(h1, i, 64);
+  h2 = fmaxf(h2, 0.f);
+#pragma unroll
+  for (int o = 0; o < kOut; ++o) {
+    float p = h2 * bf16(w[kW3 + lane * kOut + o]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
+    y[o] = p + bf16(w[kB3 + o]);
+  }
+}
+
+__device__ __forceinline__ unsigned hash32(unsigned x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Direction-optimising BFS rule (top-down -> bottom-up when the frontier's
+// edges exceed the unexplored edges / 14; back when the frontier falls below
+// n / 24); SSSP pulls when the frontier carries over a quarter of the edges.
+// Narrow when the frontier has fewer edges than the narrow groups have
+// threads.
+__device__ inline int heuristic(const FeatIn& a, unsigned valid) {
+  bool pull = false;
+  if (valid & (1u << GDB_ARM_PULL)) {
+    if (a.algo == GDB_TASK_BFS)
+      pull = arm_pull(a.prev_arm) ? a.nin >= a.n / 24.f : a.fe > a.unv_e / 14.f;
+    else
+      pull = a.fe > a.m * 0.25f;
+  }
+  if (pull) return GDB_ARM_PULL;
+  if ((valid & (1u << GDB_ARM_PUSH_NARROW)) && a.fe < a.narrow_threads) return GDB_ARM_PUSH_NARROW;
+  return GDB_ARM_PUSH;
+}
+
+// The arm of the next iteration.  `lw` is the weight copy in LDS (NULL when
+// the MLP is not in use).  Uniform across the wave.
+__device__ inline int decide(const Cfg& c, const unsigned short* lw, const FeatIn& a, unsigned valid,
+                             unsigned salt, float xi) {
+  switch (c.mode) {
+    case GDB_SEL_FIXED:
+      return (c.fixed_arm >= 0 && c.fixed_arm < kOut && (valid & (1u << c.fixed_arm))) ? c.fixed_arm
+                                                                                     : GDB_ARM_PUSH;
+    case GDB_SEL_RANDOM: {
+      const int cnt = __popc(valid);
+      int pick = (int)(hash32(c.seed * 0x9e3779b9u ^ salt) % (unsigned)cnt);
+      for (int a2 = 0; a2 < kOut; ++a2)
+        if (valid & (1u << a2)) {
+          if (pick == 0) return a2;
+          --pick;
+        }
+      return GDB_ARM_PUSH;
+    }
+    case GDB_SEL_HEURISTIC: return heuristic(a, valid);
+    case GDB_SEL_MLP: {
+      if (!lw) return heuristic(a, valid);
+      float y[kOut];
+      mlp_wave(lw, xi, y);
+      int best = GDB_ARM_PUSH;
+      for (int a2 = 1; a2 < kOut; ++a2)
+        if ((valid & (1u << a2)) && y[a2] < y[best]) best = a2;
+      return best;
+    }
+    default: return GDB_ARM_PUSH;
+  }
+}
+
+}  // namespace gdbsel

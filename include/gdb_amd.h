@@ -194,14 +194,96 @@ typedef struct {
   int delta;                /* SSSP near-far band width (0 = plain worklist) */
   int hops;                 /* BFS / SSSP: local hops a group may run past its share of
                                an iteration before the team barrier (0 = level-synchronous) */
+  const int* trow;          /* in-edge CSR (the transpose; the CSR itself for a symmetric
+                               graph) for pull iterations; NULL = push only */
+  const int* tcol;
+  const int* tw;            /* in-edge weights (SSSP) */
   int rounds;               /* out: levels / iterations / colour rounds */
   double time_us;           /* out: device time from task start to task end */
+  int arm_iters[4];         /* out: relaxing iterations (non-empty frontier) run with each
+                               schedule arm (GDB_ARM_*) */
 } gdb_task;
 
 /* team_groups: groups per team (0 = 1).  opts->wgs and opts->groups pick the
  * launch shape.  stats->rounds is the sum of task rounds. */
 int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups, const gdb_opts* opts,
                     gdb_stats* stats);
+
+/* ---- per-iteration schedule selector ---------------------------------------
+ * No reference counterpart: the reference fixes one schedule per binary
+ * (push worklists in lonestar bfs_port_kernel.cl / sssp_port_kernel.cl, pull
+ * in pannotia sssp_kernel.cl:79-165).  Here every BFS / SSSP iteration of a
+ * batched task picks one of four arms; all arms reach the same fixpoint, so
+ * outputs are identical whatever is chosen:
+ *   PUSH          relax the out-edges of the frontier queue, whole team
+ *   PULL          every vertex scans its in-edges (bottom-up BFS: unvisited
+ *                 vertices look for a parent on the previous level; SSSP: a
+ *                 Jacobi/Gauss-Seidel min over in-edges), whole team
+ *   *_NARROW      the same with only the team's first `narrow_groups` groups;
+ *                 the rest park on a team word until the schedule widens.
+ * The group that completes each team barrier evaluates the selector and
+ * publishes the arm with the barrier release.  MLP mode evaluates a
+ * 16-64-64-4 network (bf16 weights, fp32 accumulation) whose outputs predict
+ * log2 of the iteration's time in microseconds per arm, and takes the valid
+ * arm with the smallest prediction. */
+#define GDB_SEL_NONE 0       /* PUSH on the whole team every iteration (the plain schedule) */
+#define GDB_SEL_FIXED 1      /* fixed_arm every iteration (when valid) */
+#define GDB_SEL_HEURISTIC 2  /* direction-optimising rule (alpha 14, beta 24) + narrow when the
+                                frontier has fewer edges than the narrow groups have threads */
+#define GDB_SEL_RANDOM 3     /* uniformly random valid arm (trace collection) */
+#define GDB_SEL_MLP 4        /* smallest predicted cost; the heuristic when weights == NULL */
+
+#define GDB_ARM_PUSH 0
+#define GDB_ARM_PULL 1
+#define GDB_ARM_PUSH_NARROW 2
+#define GDB_ARM_PULL_NARROW 3
+
+#define GDB_MLP_IN 16
+#define GDB_MLP_HIDDEN 64
+#define GDB_MLP_OUT 4
+/* Parameter vector: W1[16][64], b1[64], W2[64][64], b2[64], W3[64][4], b3[4]
+ * (input-major: layer output j = b[j] + sum_i W[i][j] * x[i]). */
+#define GDB_MLP_PARAMS 5508
+/* Trace record (floats): features[16], arm, cost_us (-1 until the iteration
+ * ends), task index, iteration. */
+#define GDB_TRACE_STRIDE 20
+
+typedef struct {
+  int mode;                       /* GDB_SEL_* */
+  int fixed_arm;                  /* GDB_SEL_FIXED */
+  int narrow_groups;              /* groups of a narrow iteration (0 = 1) */
+  unsigned seed;                  /* GDB_SEL_RANDOM */
+  const unsigned short* weights;  /* device bf16 parameters (GDB_MLP_PARAMS) or NULL */
+  float* trace;                   /* device trace records or NULL */
+  int trace_capacity;             /* records that fit in `trace` */
+  int* trace_count;               /* device counter; may end above the capacity */
+} gdb_selector;
+
+/* gdb_batch_solve with a selector for the BFS / SSSP tasks (sel == NULL is
+ * GDB_SEL_NONE).  Tasks with hops > 0 always push.  Selector modes other
+ * than NONE need n < 2^30 per task. */
+int gdb_batch_solve_sel(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups,
+                        const gdb_opts* opts, const gdb_selector* sel, gdb_stats* stats);
+
+/* ---- MLP forward / backward / training (device buffers) ----------------- */
+/* y[count][4] = MLP(x[count][16]) with bf16 weights. */
+int gdb_mlp_forward(gdb_ctx* ctx, const unsigned short* w_bf16, const float* x, int count, float* y);
+/* Loss and gradient of L = mean over records with cost > 0 of
+ * (y[arm] - log2(cost_us))^2, evaluated with bf16 weights; grad has
+ * GDB_MLP_PARAMS floats.  *used = records with cost > 0 (host pointers for
+ * loss and used). */
+int gdb_mlp_grad(gdb_ctx* ctx, const unsigned short* w_bf16, const float* records, int count,
+                 float* grad, float* loss, int* used);
+/* Rounds fp32 parameters to bf16 (round to nearest even). */
+int gdb_mlp_to_bf16(gdb_ctx* ctx, const float* params, unsigned short* w_bf16, int count);
+/* Adam on the fp32 master parameters (m, v: GDB_MLP_PARAMS floats each, zero
+ * at the start): `steps` steps over minibatches of `batch` records drawn by
+ * a counter-based generator from `seed`; forward / backward use the bf16
+ * copy, which is refreshed after every step.  *loss = full-set loss at the
+ * end (host pointer). */
+int gdb_mlp_train(gdb_ctx* ctx, float* params, float* adam_m, float* adam_v, unsigned short* w_bf16,
+                  const float* records, int count, int steps, int batch, float lr, unsigned seed,
+                  float* loss);
 
 #ifdef __cplusplus
 }

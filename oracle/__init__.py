@@ -50,6 +50,8 @@ def lib():
             "orc_bc_ref_f32": (I, [I, P, P, P, P, P, I, P]),
             "orc_block_scan_ref": (I, [I, P, P]),
             "orc_dp_model": (I, [I, I, I, I, I, I, I, C.POINTER(I), P, C.POINTER(C.c_longlong)]),
+            "orc_mlp_forward": (None, [P, P, P]),
+            "orc_mlp_grad": (I, [P, P, I, P, C.POINTER(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)
@@ -194,3 +196,29 @@ def dp_model(groups, resident, lock=0, bar=1, skip=0, barriers=8, watchdog_ms=20
     dead = lib().orc_dp_model(groups, resident, lock, bar, skip, barriers, watchdog_ms, C.byref(p),
                               _p(ids), C.byref(bad))
     return {"deadlock": bool(dead), "P": p.value, "ids": ids, "violations": bad.value}
+
+
+MLP_PARAMS = 5508
+
+
+def mlp_forward(w, x):
+    """FP64 forward of the selector MLP for rows x[k][16]; w = parameter values."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 16)
+    y = np.empty((x.shape[0], 4), dtype=np.float64)
+    for r in range(x.shape[0]):
+        xr = np.ascontiguousarray(x[r])
+        yr = np.empty(4, dtype=np.float64)
+        lib().orc_mlp_forward(_p(w), _p(xr), _p(yr))
+        y[r] = yr
+    return y
+
+
+def mlp_grad(w, records):
+    """(grad[5508], loss, used) of the trace loss in FP64."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    rec = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 20)
+    g = np.empty(MLP_PARAMS, dtype=np.float64)
+    loss = C.c_double(0)
+    used = lib().orc_mlp_grad(_p(w), _p(rec), rec.shape[0], _p(g), C.byref(loss))
+    return g, loss.value, used

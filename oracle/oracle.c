@@ -877,3 +877,82 @@ int orc_dp_model(int groups, int resident, int lock, int bar, int skip, int barr
   free(m);
   return deadlock;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Schedule selector MLP (no reference counterpart)                          */
+/* ------------------------------------------------------------------------ */
+enum { MIN = 16, MHID = 64, MOUT = 4, MSTRIDE = 20 };
+enum {
+  MW1 = 0,
+  MB1 = MW1 + MIN * MHID,
+  MW2 = MB1 + MHID,
+  MB2 = MW2 + MHID * MHID,
+  MW3 = MB2 + MHID,
+  MB3 = MW3 + MHID * MOUT,
+  MPARAMS = MB3 + MOUT
+};
+
+static void mlp_fwd(const double* w, const double* x, double* a1, double* h1, double* a2, double* h2,
+                    double* y) {
+  for (int j = 0; j < MHID; ++j) {
+    double a = w[MB1 + j];
+    for (int i = 0; i < MIN; ++i) a += w[MW1 + i * MHID + j] * x[i];
+    a1[j] = a;
+    h1[j] = a > 0 ? a : 0;
+  }
+  for (int j = 0; j < MHID; ++j) {
+    double a = w[MB2 + j];
+    for (int i = 0; i < MHID; ++i) a += w[MW2 + i * MHID + j] * h1[i];
+    a2[j] = a;
+    h2[j] = a > 0 ? a : 0;
+  }
+  for (int o = 0; o < MOUT; ++o) {
+    double a = w[MB3 + o];
+    for (int i = 0; i < MHID; ++i) a += w[MW3 + i * MOUT + o] * h2[i];
+    y[o] = a;
+  }
+}
+
+void orc_mlp_forward(const double* w, const double* x, double* y) {
+  double a1[MHID], h1[MHID], a2[MHID], h2[MHID];
+  mlp_fwd(w, x, a1, h1, a2, h2, y);
+}
+
+int orc_mlp_grad(const double* w, const float* records, int count, double* grad, double* loss) {
+  double a1[MHID], h1[MHID], a2[MHID], h2[MHID], y[MOUT], x[MIN], dh2[MHID];
+  memset(grad, 0, sizeof(double) * MPARAMS);
+  double lsum = 0;
+  int used = 0;
+  for (int r = 0; r < count; ++r) {
+    const float* R = records + (size_t)r * MSTRIDE;
+    const double cost = R[MIN + 1];
+    const int arm = (int)R[MIN];
+    if (!(cost > 0) || arm < 0 || arm >= MOUT) continue;
+    for (int i = 0; i < MIN; ++i) x[i] = R[i];
+    mlp_fwd(w, x, a1, h1, a2, h2, y);
+    const double err = y[arm] - log2(cost);
+    lsum += err * err;
+    ++used;
+    const double dy = 2 * err;
+    grad[MB3 + arm] += dy;
+    for (int i = 0; i < MHID; ++i) {
+      grad[MW3 + i * MOUT + arm] += h2[i] * dy;
+      dh2[i] = a2[i] > 0 ? w[MW3 + i * MOUT + arm] * dy : 0;
+    }
+    for (int j = 0; j < MHID; ++j) {
+      grad[MB2 + j] += dh2[j];
+      for (int i = 0; i < MHID; ++i) grad[MW2 + i * MHID + j] += h1[i] * dh2[j];
+    }
+    for (int i = 0; i < MHID; ++i) {
+      double d = 0;
+      for (int j = 0; j < MHID; ++j) d += w[MW2 + i * MHID + j] * dh2[j];
+      const double dh1 = a1[i] > 0 ? d : 0;
+      grad[MB1 + i] += dh1;
+      for (int k = 0; k < MIN; ++k) grad[MW1 + k * MHID + i] += x[k] * dh1;
+    }
+  }
+  if (used > 0)
+    for (int p = 0; p < MPARAMS; ++p) grad[p] /= used;
+  *loss = used > 0 ? lsum / used : 0;
+  return used;
+}

@@ -87,6 +87,16 @@ int orc_block_scan_ref(int n_items, const int* in, int* out);
 int orc_dp_model(int groups, int resident, int lock, int bar, int skip, int barriers,
                  int watchdog_ms, int* participating, int* ids, long long* violations);
 
+/* ---- schedule selector MLP (FP64) ----------------------------------------
+ * No reference counterpart.  16-64-64-4 ReLU network, parameters in the
+ * library's input-major layout: W1[16][64], b1[64], W2[64][64], b2[64],
+ * W3[64][4], b3[4]; layer output j = b[j] + sum_i W[i][j] x[i]. */
+void orc_mlp_forward(const double* w, const double* x, double* y);
+/* Loss L = mean over trace records (20 floats each: features[16], arm,
+ * cost_us, ...) with cost > 0 of (y[arm] - log2(cost))^2, and dL/dw
+ * (5508 doubles) by backpropagation.  Returns the number of records used. */
+int orc_mlp_grad(const double* w, const float* records, int count, double* grad, double* loss);
+
 #ifdef __cplusplus
 }
 #endif
