@@ -104,18 +104,33 @@ def main() -> int:
     out_s = gdb.DeviceBuffer.empty(g.n, np.uint32)
     last_tasks = []
 
-    def solve_batched(wgs, groups, delta, hops=0):
-        # One launch, two teams: BFS and near/far SSSP side by side.
-        tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0, hops=hops),
-                 gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta, hops=hops)]
+    # The learned schedule selector: the MLP trained on the engine's own
+    # traces (gpu_discovery_barrier_amd/selector_weights.npy).
+    wpath = ROOT / "gpu_discovery_barrier_amd" / "selector_weights.npy"
+    weights = gdb.DeviceBuffer.from_numpy(np.load(wpath).astype(np.uint16)) if wpath.exists() else None
+    narrow = 2
+
+    def selector(mode):
+        if mode == gdb.SEL_NONE:
+            return None
+        return gdb.make_selector(mode, narrow_groups=narrow, weights=weights)
+
+    def solve_batched(wgs, groups, delta, hops=0, mode=gdb.SEL_NONE):
+        # One launch, two teams: BFS and near/far SSSP side by side.  The
+        # grid is symmetric, so it is its own in-edge CSR for pull arms.
+        tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0, hops=hops, transpose=dg),
+                 gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta, hops=hops, transpose=dg)]
         rounds, st = gdb.batch_solve(ctx, tasks, max(1, groups // 2),
-                                     gdb.Opts.make(wgs=wgs, groups=groups))
+                                     gdb.Opts.make(wgs=wgs, groups=groups), selector=selector(mode))
         last_tasks[:] = tasks
         return rounds, st
 
-    # Launch-shape / band-width autotuning of the batched engine (untimed).
+    # Autotuning of the batched engine (untimed): launch shape, near/far
+    # band, local hops, and whether the learned selector schedules the
+    # iterations (it needs hops = 0).
+    modes = [gdb.SEL_NONE] + ([gdb.SEL_MLP] if weights is not None else [])
     if args.no_tune:
-        wgs, groups, delta, hops = 256, 64, 4096, 8
+        wgs, groups, delta, hops, mode = 256, 64, 4096, 0, modes[-1]
     else:
         trials = []
         for w in (256, 512, 1024):
@@ -123,19 +138,22 @@ def main() -> int:
                 gr = min(gr, gdb.max_resident_groups(w, 0))
                 for dl in (2048, 4096, 8192):
                     for hp in (0, 4, 16):
-                        _, st = solve_batched(w, gr, dl, hp)
-                        trials.append((st.kernel_ms, w, gr, dl, hp))
+                        for md in modes:
+                            if md != gdb.SEL_NONE and hp > 0:
+                                continue
+                            _, st = solve_batched(w, gr, dl, hp, md)
+                            trials.append((st.kernel_ms, w, gr, dl, hp, md))
         trials.sort()
-        _, wgs, groups, delta, hops = trials[0]
-        log(f"autotune: wgs={wgs} groups={groups} delta={delta} hops={hops} ({trials[0][0]:.2f} ms); "
-            f"{len(trials)} shapes tried")
+        _, wgs, groups, delta, hops, mode = trials[0]
+        log(f"autotune: wgs={wgs} groups={groups} delta={delta} hops={hops} selector={mode} "
+            f"({trials[0][0]:.2f} ms); {len(trials)} shapes tried")
 
     for _ in range(args.warmup):
-        solve_batched(wgs, groups, delta, hops)
+        solve_batched(wgs, groups, delta, hops, mode)
     step_ms, walls = [], []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        rounds, bst = solve_batched(wgs, groups, delta, hops)
+        rounds, bst = solve_batched(wgs, groups, delta, hops, mode)
         walls.append(time.perf_counter() - t0)
         step_ms.append(bst.kernel_ms)
     db, ds = out_b.to_numpy(), out_s.to_numpy()
@@ -193,6 +211,9 @@ def main() -> int:
             "engine": "batched (BFS and near/far SSSP teams in one persistent launch)",
             "wgs": wgs, "groups": groups, "team_groups": max(1, groups // 2), "delta": delta,
             "local_hops": hops,
+            "selector": {gdb.SEL_NONE: "none (push on the whole team)", gdb.SEL_MLP: "learned MLP"}[mode],
+            "narrow_groups": narrow,
+            "arm_iterations": {"bfs": list(last_tasks[0].arm_iters), "sssp": list(last_tasks[1].arm_iters)},
             "participating": bst.participating, "bfs_iterations": rounds[0], "sssp_iterations": rounds[1],
         },
         "task_ms": {"bfs": round(last_tasks[0].time_us / 1e3, 3), "sssp": round(last_tasks[1].time_us / 1e3, 3)},

@@ -34,17 +34,17 @@ constexpr int kSerialLimit = 16;
 constexpr int kChunk = 4;                  // edges a lane relaxes per step
 constexpr int kLine = GDB_LINE_BYTES / 4;  // ints per 128 B line
 
-// Team state, one 128 B line per word.  TW_RESUME, TW_SEL and TW_CUM are
-// lines of several words (see below).
+// Team state, one 128 B line per word.  TW_RESUME and TW_SEL are lines of
+// several words (see below).
 enum TeamWord { TW_COUNT = 0, TW_GEN, TW_TASK, TW_TAIL0, TW_TAIL1, TW_TAIL2, TW_FAR0, TW_FAR1, TW_FAR2,
-                TW_STOP0, TW_STOP1, TW_STOP2, TW_PARK, TW_RESUME, TW_SEL, TW_CUM, TW_WORDS };
+                TW_STOP0, TW_STOP1, TW_STOP2, TW_PARK, TW_RESUME, TW_SEL, TW_WORDS };
 // TW_RESUME line: what a parked group needs to rejoin (written before the
 // release of TW_PARK).
-enum ResumeWord { RS_DONE = 0, RS_IT, RS_PAYLOAD, RS_FGEN, RS_THRESH, RS_BARGEN, RS_WORDS };
-// TW_SEL line (ints): [0,1] cumulative frontier edges at the last release,
-// [2,3] time of the last release, [4] trace record awaiting its cost.
-// TW_CUM line (u64): [0] out-edges of vertices pushed to the team queue,
-// [1] vertices reached, [2] out-edges of vertices reached.
+enum ResumeWord {
+  RS_DONE = 0, RS_IT, RS_PAYLOAD, RS_FGEN, RS_THRESH, RS_BARGEN, RS_TOTAL, RS_EVAL_NIN, RS_EVAL_IT, RS_WORDS
+};
+// TW_SEL line (ints, trace mode only): [0,1] time of the last release, [2]
+// trace record awaiting its cost.
 constexpr int kNinBits = 30;  // barrier payload = frontier size | arm << 30
 constexpr int kNinMask = (1 << kNinBits) - 1;
 
@@ -70,12 +70,25 @@ struct SelRun {
   unsigned valid;  // arms this task can run
 };
 
-// Loop state a releasing group publishes to parked groups.
+// Loop state of the active groups (identical in each of them): what the
+// releasing group decides from and publishes to parked groups.  `total` is
+// the sum of the frontier sizes so far (for BFS, the vertices reached);
+// eval_nin / eval_it are the frontier size and iteration of the last MLP
+// evaluation.
 struct IterState {
   int it, arm, nin, fgen;
   unsigned thresh;
-  int wakes;
+  int wakes, total, eval_nin, eval_it;
 };
+
+// An MLP decision is reused while the frontier stays within 25% of the size
+// it had when the network last ran, for at most 32 iterations, so that long
+// runs of similar iterations (a grid's wavefront) do not pay the inference
+// every time.  Every active group evaluates this on the same values.
+__device__ __forceinline__ bool mlp_reuse(const IterState& S, int nin) {
+  return S.eval_it > 0 && S.it + 1 - S.eval_it < 32 && nin > 0 &&
+         (long long)nin * 4 <= (long long)S.eval_nin * 5 && (long long)nin * 5 >= (long long)S.eval_nin * 4;
+}
 
 constexpr int kLocalQ = 2048;  // LDS queue capacity per buffer (ints)
 
@@ -145,6 +158,10 @@ __device__ inline int team_barrier_payload(Team& T, const int* src, int* lds_val
   return *lds_val;
 }
 
+__device__ __forceinline__ int sat_add(int a, int b) {
+  const long long s = (long long)a + b;
+  return s > 0x7fffffff ? 0x7fffffff : (int)s;
+}
 __device__ __forceinline__ unsigned long long ld64(const void* p) {
   return __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -152,38 +169,38 @@ __device__ __forceinline__ void st64(void* p, unsigned long long v) {
   __hip_atomic_store((unsigned long long*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Run by wave 0 of the group that completes a team barrier: fills in the
-// cost of the iteration that just ended in its trace record, computes the
-// next iteration's features and arm, appends its trace record, and returns
-// the payload (frontier size | arm << 30).
+// Run by wave 0 of the group that completes a team barrier: decides the
+// next iteration's arm and returns the payload (frontier size | arm << 30).
+// The features come from the frontier size just read and the loop state
+// every active group carries, so the decision costs no extra memory round
+// trip; only the trace mode reads and writes the TW_SEL line (the time of
+// the last release and the record awaiting its cost).
 __device__ int sel_release(Team& T, const int* tail, const SelRun& R, const DevTask& k,
                            const IterState& S) {
   const int lane = (int)__lane_id();
-  const int* sl = T.word(TW_SEL);
-  const unsigned long long* cum = (const unsigned long long*)T.word(TW_CUM);
+  const gdbsel::Cfg& c = R.cfg;
+  const bool tracing = c.trace && c.trace_count;
+  int* sl = T.word(TW_SEL);
   // One load per lane, issued together.
   unsigned long long x = 0;
-  if (lane == 0) x = ld64(sl);
-  else if (lane == 1) x = ld64(sl + 2);
-  else if (lane == 2) x = (unsigned)gdb::ld_rlx(sl + 4);
-  else if (lane == 3) x = ld64(cum);
-  else if (lane == 4) x = ld64(cum + 1);
-  else if (lane == 5) x = ld64(cum + 2);
-  else if (lane == 6) x = (unsigned)gdb::ld_rlx(tail);
+  if (lane == 0) x = (unsigned)gdb::ld_rlx(tail);
+  else if (tracing && lane == 1) x = ld64(sl);
+  else if (tracing && lane == 2) x = (unsigned)gdb::ld_rlx(sl + 2);
   auto get = [&](int l) -> unsigned long long {
     const unsigned lo = (unsigned)__shfl((int)(unsigned)x, l, 64);
     const unsigned hi = (unsigned)__shfl((int)(unsigned)(x >> 32), l, 64);
     return ((unsigned long long)hi << 32) | lo;
   };
-  const unsigned long long prev_fe = get(0), t_rel = get(1), cum_fe = get(3), cum_rc = get(4),
-                           cum_re = get(5);
-  const int pending = (int)(unsigned)get(2);
-  const int nin = (int)(unsigned)get(6);
-  const unsigned long long t = gdb::now();
-  const gdbsel::Cfg& c = R.cfg;
-  if (c.trace && lane == 0 && pending >= 0 && pending < c.trace_capacity)
-    c.trace[(size_t)pending * gdbsel::kStride + gdbsel::kIn + 1] = (float)(t - t_rel) * 0.01f;
+  const int nin = (int)(unsigned)get(0);
   int arm, rec = -1;
+  unsigned long long t = 0;
+  if (tracing) {
+    t = gdb::now();
+    const unsigned long long t_rel = get(1);
+    const int pending = (int)(unsigned)get(2);
+    if (lane == 0 && pending >= 0 && pending < c.trace_capacity)
+      c.trace[(size_t)pending * gdbsel::kStride + gdbsel::kIn + 1] = (float)(t - t_rel) * 0.01f;
+  }
   if (nin == 0) {
     // A far-pile split (or the end): keep the team's width, nothing to learn.
     arm = gdbsel::arm_narrow(S.arm) ? GDB_ARM_PUSH_NARROW : GDB_ARM_PUSH;
@@ -193,17 +210,20 @@ __device__ int sel_release(Team& T, const int* tail, const SelRun& R, const DevT
     a.n = (float)k.n;
     a.m = (float)k.m;
     a.nin = (float)nin;
-    a.fe = (float)(cum_fe - prev_fe);
-    a.unv = (float)((long long)k.n - (long long)cum_rc);
-    a.unv_e = (float)((long long)k.m - (long long)cum_re);
+    a.total = (float)S.total + (float)nin;
     a.prev_nin = (float)S.nin;
+    a.it = S.it + 1;
     a.algo = k.algo;
     a.prev_arm = S.arm;
     a.team_threads = (float)(T.G * (int)blockDim.x);
     a.narrow_threads = (float)(R.Gn * (int)blockDim.x);
-    const float xi = lane < gdbsel::kIn ? gdbsel::feature(a, lane) : 0.f;
-    arm = gdbsel::decide(c, R.lw, a, R.valid, (unsigned)R.task * 0x10001u + (unsigned)S.it, xi);
-    if (c.trace && c.trace_count) {
+    // The MLP input (lane i holds feature i); only the MLP and the trace use it.
+    const bool use_mlp = c.mode == GDB_SEL_MLP && R.lw;
+    const bool reuse = use_mlp && mlp_reuse(S, nin);
+    const bool need_x = (use_mlp && !reuse) || tracing;
+    const float xi = (need_x && lane < gdbsel::kIn) ? gdbsel::feature(a, lane) : 0.f;
+    arm = reuse ? S.arm : gdbsel::decide(c, R.lw, a, R.valid, (unsigned)R.task * 0x10001u + (unsigned)S.it, xi);
+    if (tracing) {
       if (lane == 0) rec = atomicAdd(c.trace_count, 1);
       rec = __shfl(rec, 0, 64);
       if (rec < c.trace_capacity) {
@@ -220,11 +240,9 @@ __device__ int sel_release(Team& T, const int* tail, const SelRun& R, const DevT
       }
     }
   }
-  if (lane == 0) {
-    int* slw = T.word(TW_SEL);
-    st64(slw, cum_fe);
-    st64(slw + 2, t);
-    gdb::st_rlx(slw + 4, rec);
+  if (tracing && lane == 0) {
+    st64(sl, t);
+    gdb::st_rlx(sl + 2, rec);
   }
   return nin | (arm << kNinBits);
 }
@@ -272,6 +290,10 @@ __device__ int team_barrier_sel(Team& T, int members, const int* tail, const Sel
         gdb::st_rlx(rs + RS_FGEN, S.fgen);
         gdb::st_rlx(rs + RS_THRESH, (int)S.thresh);
         gdb::st_rlx(rs + RS_BARGEN, ng);
+        gdb::st_rlx(rs + RS_TOTAL, sat_add(S.total, val & kNinMask));
+        const bool ev = R.cfg.mode == GDB_SEL_MLP && R.lw && (val & kNinMask) > 0 && !mlp_reuse(S, val & kNinMask);
+        gdb::st_rlx(rs + RS_EVAL_NIN, ev ? (val & kNinMask) : S.eval_nin);
+        gdb::st_rlx(rs + RS_EVAL_IT, ev ? S.it + 1 : S.eval_it);
       }
       if (cross) {
         const unsigned long long w = ((unsigned long long)(unsigned)val << 32) | (unsigned)ng;
@@ -307,8 +329,8 @@ __device__ bool team_park(Team& T, int wakes, int (&rs)[RS_WORDS]) {
   __syncthreads();
   if (threadIdx.x == 0) {
     const int* pw = T.word(TW_PARK);
-    const bool ok = gdb::wait_while(T.ctx, [&] { return gdb::ld_rlx(pw) <= wakes; },
-                                    GDB_ERR_BARRIER_TIMEOUT);
+    const bool ok = gdb::wait_while_idle(T.ctx, [&] { return gdb::ld_rlx(pw) <= wakes; },
+                                         GDB_ERR_BARRIER_TIMEOUT);
     gdb::fence_acq();
     s_ok = ok;
     if (!ok) T.l->aborted = 1;
@@ -340,8 +362,8 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, long long v) {
 // hops; what is left spills to the team's queue.  Any relaxation order
 // reaches the same fixpoint (exact levels / distances).
 template <bool WEIGHTED, bool UNIT, bool SEL>
-__device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work, const SelRun& R,
-                             int (&arm_iters)[4]) {
+__device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work,
+                                             const SelRun& R) {
   // scratch: BFS (CAS)  q0[n], q1[n]
   //          otherwise  d64[n] (8 B each), fstamp[n], gstamp[n], q0[n], q1[n], far0..2[n]
   // Marks in the low word of d64 (level-synchronous iterations only):
@@ -388,15 +410,9 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
       gdb::st_rlx(T.word(TW_FAR1), 0);
       gdb::st_rlx(T.word(TW_FAR2), 0);
       if (sel) {
-        const unsigned long long d0 = (unsigned long long)(k.row[k.source + 1] - k.row[k.source]);
         int* sl = T.word(TW_SEL);
-        unsigned long long* cum = (unsigned long long*)T.word(TW_CUM);
-        st64(sl, 0);
-        st64(sl + 2, gdb::now());
-        gdb::st_rlx(sl + 4, -1);
-        st64(cum, d0);
-        st64(cum + 1, 1);
-        st64(cum + 2, d0);
+        st64(sl, gdb::now());
+        gdb::st_rlx(sl + 2, -1);
       }
     }
   }
@@ -406,17 +422,25 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
   int nin = 1;
   int arm = GDB_ARM_PUSH;
   int wakes = 0;  // wake-ups of parked groups seen so far
+  int total = 0;  // sum of frontier sizes so far
+  int eval_nin = 0, eval_it = 0;
+  const bool use_mlp = R.cfg.mode == GDB_SEL_MLP && R.lw;
   if (SEL && sel) {
-    const IterState S0{0, GDB_ARM_PUSH, 0, 0, thresh, 0};
+    const IterState S0{0, GDB_ARM_PUSH, 0, 0, thresh, 0, 0, 0, 0};
     const int pl = team_barrier_sel(T, G, T.word(TW_TAIL1), R, k, S0);
     if (T.l->aborted) return -1;
     nin = pl & kNinMask;
     arm = (int)((unsigned)pl >> kNinBits);
+    total = nin;
+    if (use_mlp) {
+      eval_nin = nin;
+      eval_it = 1;
+    }
   } else if (!team_barrier(T)) {
     return -1;
   }
   long long items = 0, edges = 0;
-  long long f_fe = 0, f_rc = 0, f_re = 0;  // selector features (this thread's share)
+  int ai[4] = {0, 0, 0, 0};
   for (;;) {
     const int width = gdbsel::arm_narrow(arm) ? Gn : G;
     if (SEL && T.rank >= width) {
@@ -430,10 +454,18 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
       arm = (int)((unsigned)rs[RS_PAYLOAD] >> kNinBits);
       gen = rs[RS_FGEN];
       thresh = (unsigned)rs[RS_THRESH];
+      total = rs[RS_TOTAL];
+      eval_nin = rs[RS_EVAL_NIN];
+      eval_it = rs[RS_EVAL_IT];
       continue;
     }
     const int tid = T.rank * (int)blockDim.x + (int)threadIdx.x, stride = width * (int)blockDim.x;
-    if (T.rank == 0 && threadIdx.x == 0 && nin > 0) ++arm_iters[arm & 3];  // relaxing iterations
+    if (nin > 0) {  // relaxing iterations per arm (kept in scalar registers)
+      ai[0] += arm == 0;
+      ai[1] += arm == 1;
+      ai[2] += arm == 2;
+      ai[3] += arm == 3;
+    }
     const int* in = q[(it - 1) & 1];
     int* outq = q[it & 1];
     int* out_tail = T.word(TW_TAIL0 + (it + 1) % 3);
@@ -441,17 +473,6 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
     const unsigned near_mark = 2u * (unsigned)it, far_mark = near_mark + 1u;
     int* fpile = far[gen % 3];
     int* ftail = T.word(TW_FAR0 + gen % 3);
-    // Selector features of a vertex entering the next frontier (to_team) or
-    // reached for the first time.
-    auto account = [&](int v, bool to_team, bool first) {
-      if (!sel || !(to_team || first) || (unsigned)v >= (unsigned)n) return;
-      const long long d = k.row[v + 1] - k.row[v];
-      if (to_team) f_fe += d;
-      if (first) {
-        ++f_rc;
-        f_re += d;
-      }
-    };
     if (SEL && !UNIT && nin > 0 && gdbsel::arm_pull(arm)) {
       if (nin > n) {
         gdb::set_error(T.ctx, GDB_ERR_QUEUE_OVERFLOW);
@@ -500,9 +521,7 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
                 [](unsigned x, unsigned y) { return x < y ? x : y; }, kSerialLimit);
           }
           int cls = 0;
-          bool first = false;
           if (valid && best < cur) {
-            first = cur == kInf;
             if (!WEIGHTED) {
               dist[v] = best;
               cls = 1;
@@ -512,7 +531,6 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
               cls = nearp ? 1 : (atomicExch(&fstamp[v], gen) != gen ? 2 : 0);
             }
           }
-          account(v, cls == 1, first);
           gdb::wave_push(cls == 1, v, outq, out_tail);
           if (near_far) gdb::wave_push(cls == 2, v, fpile, ftail);
         }
@@ -525,13 +543,12 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
       if (hops > 0 && threadIdx.x < 3) lcnt[threadIdx.x] = 0;
       if (hops > 0) __syncthreads();
       // Relaxes edge kk from a vertex at distance src_d.  Returns the push
-      // class (0 none, 1 team queue, 2 far pile, 3 local LDS queue), plus 4
-      // when dst was reached for the first time.
+      // class: 0 none, 1 team queue, 2 far pile, 3 local (LDS) queue.
       auto relax = [&](int kk, unsigned src_d, int* dst_out) -> int {
         const int dst = k.col[kk];
         *dst_out = dst;
         if ((unsigned)dst >= (unsigned)n) return 0;
-        if (!WEIGHTED) return atomicCAS(&dist[dst], kInf, src_d) == kInf ? 1 | 4 : 0;
+        if (!WEIGHTED) return atomicCAS(&dist[dst], kInf, src_d) == kInf ? 1 : 0;
         const unsigned wt = UNIT ? 1u : (unsigned)k.w[kk];
         if (wt >= kInf) return 0;
         const unsigned cand = src_d + wt;
@@ -544,11 +561,9 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
         }
         const unsigned long long nw = ((unsigned long long)cand << 32) | (nearp ? near_mark : far_mark);
         const unsigned long long old = atomicMin(&d64[dst], nw);
-        const unsigned oldd = (unsigned)(old >> 32);
-        if (cand >= oldd) return 0;
-        const int fr = oldd == kInf ? 4 : 0;
-        if (nearp) return ((unsigned)old == near_mark ? 0 : 1) | fr;
-        return (atomicExch(&fstamp[dst], gen) != gen ? 2 : 0) | fr;
+        if (cand >= (unsigned)(old >> 32)) return 0;
+        if (nearp) return (unsigned)old == near_mark ? 0 : 1;
+        return atomicExch(&fstamp[dst], gen) != gen ? 2 : 0;
       };
       // Pushes the results of one relaxation step of this wave.  Class 3
       // goes to LDS buffer `lb` when there is room, else to the team queue.
@@ -556,7 +571,7 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
         bool to_team[kChunk], to_far[kChunk];
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
-          const int c = cls[j] & 3;
+          const int c = cls[j];
           bool team_push = c == 1;
           if (c == 3) {
             const int pos = atomicAdd(lc, 1);
@@ -565,7 +580,6 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
           }
           to_team[j] = team_push;
           to_far[j] = c == 2;
-          account(dst[j], team_push, (cls[j] & 4) != 0);
         }
         gdb::wave_push_n<kChunk>(to_team, dst, outq, out_tail);
         if (near_far) gdb::wave_push_n<kChunk>(to_far, dst, fpile, ftail);
@@ -672,25 +686,24 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
             else to_far = atomicExch(&fstamp[v], gen + 1) != gen + 1;
           }
         }
-        account(v, to_near, false);
         gdb::wave_push(to_near, v, outq, out_tail);
         gdb::wave_push(to_far, v, next_pile, next_tail);
       }
       ++gen;
     }
     if (SEL && sel) {
-      unsigned long long* cum = (unsigned long long*)T.word(TW_CUM);
-      wave_add(cum, f_fe);
-      wave_add(cum + 1, f_rc);
-      wave_add(cum + 2, f_re);
-      f_fe = f_rc = f_re = 0;
-      const IterState S{it, arm, nin, gen, thresh, wakes};
+      const IterState S{it, arm, nin, gen, thresh, wakes, total, eval_nin, eval_it};
       const int pl = team_barrier_sel(T, width, out_tail, R, k, S);
       if (T.l->aborted) return -1;
       const int new_arm = (int)((unsigned)pl >> kNinBits);
       if (width < G && !gdbsel::arm_narrow(new_arm)) ++wakes;  // the releaser woke the parked groups
       nin = pl & kNinMask;
       arm = new_arm;
+      total = sat_add(total, nin);
+      if (use_mlp && nin > 0 && !mlp_reuse(S, nin)) {  // the releaser ran the network
+        eval_nin = nin;
+        eval_it = it + 1;
+      }
     } else {
       nin = team_barrier_payload(T, out_tail, &lds_n);
       if (T.l->aborted) return -1;
@@ -701,12 +714,14 @@ __device__ int team_worklist(Team& T, const DevTask& k, unsigned long long* work
     for (int v = T.tid(); v < n; v += T.stride()) dist[v] = (unsigned)(d64[v] >> 32);
   wave_add(&work[0], items);
   wave_add(&work[1], edges);
+  if (T.rank == 0 && threadIdx.x == 0)
+    for (int a = 0; a < 4; ++a) work[4 + a] = (unsigned long long)ai[a];
   return it - 1;
 }
 
 // ---- colour on a team (one barrier per round, as k_color_tuned) ------------
 // scratch: maxd[2n] (floats)
-__device__ int team_color(Team& T, const DevTask& k) {
+__device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
   const int n = k.n;
   int* color = (int*)k.out;
   float* maxd = (float*)k.scratch;
@@ -802,26 +817,30 @@ __global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevT
     const DevTask k = tasks[t];
     if (T.tid() == 0) work[8 * (size_t)t + 2] = gdb::now();
     int rounds = 0;
-    int arm_iters[4] = {0, 0, 0, 0};
     if (k.n > 0) {
       unsigned long long* wk = work + 8 * (size_t)t;
-      const bool pull_ok = k.trow && k.tcol && (k.algo == GDB_TASK_BFS || k.tw);
+      // Pull arms need the in-edge CSR.  The modes that choose (heuristic,
+      // MLP, random) offer them to BFS only: a BFS iteration reaches the
+      // same next level whichever arm runs, so the cheapest arm per
+      // iteration is the cheapest schedule; an SSSP pull iteration makes
+      // different progress than a push one, which a per-iteration cost
+      // cannot weigh.  GDB_SEL_FIXED may still pull SSSP.
+      const bool pull_ok = k.trow && k.tcol && (k.algo == GDB_TASK_BFS || (k.tw && sel.mode == GDB_SEL_FIXED));
       unsigned valid = 1u << GDB_ARM_PUSH;
       if (pull_ok) valid |= 1u << GDB_ARM_PULL;
       if (Gn < G) valid |= 1u << GDB_ARM_PUSH_NARROW;
       if (pull_ok && Gn < G) valid |= 1u << GDB_ARM_PULL_NARROW;
       const SelRun R{sel, mlp ? sel_w : nullptr, t, Gn, valid};
       if (k.algo == GDB_TASK_BFS)
-        rounds = k.hops > 0 ? team_worklist<true, true, false>(T, k, wk, R, arm_iters)
-                            : team_worklist<false, false, SEL>(T, k, wk, R, arm_iters);
-      else if (k.algo == GDB_TASK_SSSP) rounds = team_worklist<true, false, SEL>(T, k, wk, R, arm_iters);
+        rounds = k.hops > 0 ? team_worklist<true, true, false>(T, k, wk, R)
+                            : team_worklist<false, false, SEL>(T, k, wk, R);
+      else if (k.algo == GDB_TASK_SSSP) rounds = team_worklist<true, false, SEL>(T, k, wk, R);
       else rounds = team_color(T, k);
     }
     if (rounds < 0) return;
     if (T.tid() == 0) {
       results[t] = rounds;
       work[8 * (size_t)t + 3] = gdb::now();
-      for (int a = 0; a < 4; ++a) work[8 * (size_t)t + 4 + a] = (unsigned long long)arm_iters[a];
     }
     if (!team_barrier(T)) return;
   }

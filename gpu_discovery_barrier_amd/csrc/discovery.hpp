@@ -119,6 +119,36 @@ __device__ __forceinline__ bool wait_while(gdb_kernel_ctx_t* ctx, Pred pred, int
   return true;
 }
 
+// Bounded wait for long idle periods (a group parked while the rest of its
+// team works): after 32 quick polls it polls every ~8 us, so that many
+// parked groups do not load the memory system the active groups use.
+template <typename Pred>
+__device__ __forceinline__ bool wait_while_idle(gdb_kernel_ctx_t* ctx, Pred pred, int err_code) {
+  if (!pred()) return true;
+  const unsigned long long limit = (unsigned long long)(unsigned)ld_rlx(&ctx->timeout_ticks.v);
+  if (limit == 0) {
+    set_error(ctx, err_code);
+    return false;
+  }
+  const unsigned long long t0 = now();
+  unsigned it = 0;
+  while (pred()) {
+    if (it < 32) {
+      __builtin_amdgcn_s_sleep(2);
+    } else {
+      for (int k = 0; k < 8; ++k) __builtin_amdgcn_s_sleep(127);
+    }
+    if ((++it & 7u) == 0) {
+      if (ld_rlx(&ctx->error.v) != GDB_OK) return false;
+      if (now() - t0 > limit) {
+        set_error(ctx, err_code);
+        return false;
+      }
+    }
+  }
+  return true;
+}
+
 // ---- mutex variants of the reference protocol (locks.cl) -----------------
 __device__ inline bool lock_acquire(gdb_kernel_ctx_t* ctx, int kind) {
   if (kind == GDB_LOCK_SPIN)  // locks.cl:11-15

@@ -6,7 +6,7 @@
 // (y[arm] - log2(cost_us))^2, so each output learns log2 of the iteration
 // time of its arm.  One wave evaluates one record at a time: lane j holds
 // hidden unit j of both layers; inputs and activations travel by lane
-// shuffles; each wave keeps its share of the gradient in registers and adds
+// readlane broadcasts; each wave keeps its share of the gradient in registers and adds
 // it to the global sum once.
 #include <algorithm>
 
@@ -72,11 +72,11 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const unsigned short* __restri
     const float xi = lane < kIn ? R[lane] : 0.f;
     float a1 = sw[kB1 + lane];
 #pragma unroll
-    for (int i = 0; i < kIn; ++i) a1 += sw[kW1 + i * kHid + lane] * __shfl(xi, i, 64);
+    for (int i = 0; i < kIn; ++i) a1 += sw[kW1 + i * kHid + lane] * bcast(xi, i);
     const float h1 = fmaxf(a1, 0.f);
     float a2 = sw[kB2 + lane];
-#pragma unroll 16
-    for (int i = 0; i < kHid; ++i) a2 += sw[kW2 + i * kHid + lane] * __shfl(h1, i, 64);
+#pragma unroll
+    for (int i = 0; i < kHid; ++i) a2 += sw[kW2 + i * kHid + lane] * bcast(h1, i);
     const float h2 = fmaxf(a2, 0.f);
     const float w3 = sw[kW3 + lane * kOut + arm];
     const float y = wave_sum(h2 * w3) + sw[kB3 + arm];
@@ -93,14 +93,14 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const unsigned short* __restri
     const float dh2 = a2 > 0.f ? w3 * dy : 0.f;
     gb2 += dh2;
 #pragma unroll
-    for (int i = 0; i < kHid; ++i) gW2[i] += __shfl(h1, i, 64) * dh2;
+    for (int i = 0; i < kHid; ++i) gW2[i] += bcast(h1, i) * dh2;
     float d1 = 0.f;
-#pragma unroll 16
-    for (int j = 0; j < kHid; ++j) d1 += sw2[lane * kPad + j] * __shfl(dh2, j, 64);
+#pragma unroll
+    for (int j = 0; j < kHid; ++j) d1 += sw2[lane * kPad + j] * bcast(dh2, j);
     const float dh1 = a1 > 0.f ? d1 : 0.f;
     gb1 += dh1;
 #pragma unroll
-    for (int i = 0; i < kIn; ++i) gW1[i] += __shfl(xi, i, 64) * dh1;
+    for (int i = 0; i < kIn; ++i) gW1[i] += bcast(xi, i) * dh1;
   }
   if (nused == 0) return;  // uniform across the wave
 #pragma unroll

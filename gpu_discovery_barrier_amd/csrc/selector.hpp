@@ -41,12 +41,17 @@ __device__ __forceinline__ unsigned short to_bf16(float f) {
 __device__ __forceinline__ bool arm_pull(int a) { return a == GDB_ARM_PULL || a == GDB_ARM_PULL_NARROW; }
 __device__ __forceinline__ bool arm_narrow(int a) { return a >= GDB_ARM_PUSH_NARROW; }
 
-// Iteration features (all lanes compute the same values).
+// Iteration features (all lanes compute the same values).  They use only
+// counts every group of the team already holds: the next frontier's size,
+// the running sum of frontier sizes (for BFS, the vertices reached), the
+// graph's size and the team's shape; frontier edges are estimated with the
+// average degree.
 struct FeatIn {
   float n, m;           // graph size
-  float nin, fe;        // next frontier: vertices, out-edges
-  float unv, unv_e;     // vertices not reached yet, and their out-edges
+  float nin;            // next frontier size
+  float total;          // sum of frontier sizes including the next one
   float prev_nin;       // frontier of the iteration that just ended
+  int it;               // index of the next iteration
   int algo, prev_arm;
   float team_threads, narrow_threads;
 };
@@ -54,17 +59,19 @@ struct FeatIn {
 __device__ __forceinline__ float lg(float x) { return __log2f(1.f + fmaxf(x, 0.f)) * (1.f / 32.f); }
 
 __device__ __forceinline__ float feature(const FeatIn& a, int i) {
+  const float deg = a.m / fmaxf(a.n, 1.f);
+  const float unv = fmaxf(a.n - a.total, 0.f);
   switch (i) {
     case 0: return lg(a.nin);
-    case 1: return lg(a.fe);
-    case 2: return lg(a.unv);
-    case 3: return lg(a.unv_e);
+    case 1: return lg(a.nin * deg);
+    case 2: return lg(unv);
+    case 3: return lg(unv * deg);
     case 4: return lg(a.n);
     case 5: return lg(a.m);
     case 6: return a.nin / fmaxf(a.n, 1.f);
-    case 7: return a.fe / fmaxf(a.m, 1.f);
-    case 8: return a.unv / fmaxf(a.n, 1.f);
-    case 9: return a.unv_e / fmaxf(a.m, 1.f);
+    case 7: return fminf(a.total / fmaxf(a.n, 1.f), 4.f) * 0.25f;
+    case 8: return unv / fmaxf(a.n, 1.f);
+    case 9: return (float)a.it / ((float)a.it + 64.f);
     case 10: return (float)a.algo;
     case 11: return arm_pull(a.prev_arm) ? 1.f : 0.f;
     case 12: return arm_narrow(a.prev_arm) ? 1.f : 0.f;
@@ -74,17 +81,25 @@ __device__ __forceinline__ float feature(const FeatIn& a, int i) {
   }
 }
 
+// Broadcast of lane `i` (a compile-time constant after unrolling) to the
+// wave through a scalar register.
+__device__ __forceinline__ float bcast(float v, int i) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+}
+
 // y = MLP(x) for the features held by lanes 0..15 (`xi`); every lane of the
-// wave must call it and gets all four outputs.
+// wave must call it and gets all four outputs.  Lane j computes hidden unit j
+// of both layers (its weights W[i][j] are consecutive across lanes), the
+// activations travel by readlane, the output layer is a wave reduction.
 __device__ inline void mlp_wave(const unsigned short* __restrict__ w, float xi, float (&y)[kOut]) {
   const int lane = (int)__lane_id();
   float h1 = bf16(w[kB1 + lane]);
 #pragma unroll
-  for (int i = 0; i < kIn; ++i) h1 += bf16(w[kW1 + i * kHid + lane]) * __shfl(xi, i, 64);
+  for (int i = 0; i < kIn; ++i) h1 += bf16(w[kW1 + i * kHid + lane]) * bcast(xi, i);
   h1 = fmaxf(h1, 0.f);
   float h2 = bf16(w[kB2 + lane]);
-#pragma unroll 16
-  for (int i = 0; i < kHid; ++i) h2 += bf16(w[kW2 + i * kHid + lane]) * __shfl(h1, i, 64);
+#pragma unroll
+  for (int i = 0; i < kHid; ++i) h2 += bf16(w[kW2 + i * kHid + lane]) * bcast(h1, i);
   h2 = fmaxf(h2, 0.f);
 #pragma unroll
   for (int o = 0; o < kOut; ++o) {
@@ -104,21 +119,23 @@ __device__ __forceinline__ unsigned hash32(unsigned x) {
   return x;
 }
 
-// Direction-optimising BFS rule (top-down -> bottom-up when the frontier's
-// edges exceed the unexplored edges / 14; back when the frontier falls below
-// n / 24); SSSP pulls when the frontier carries over a quarter of the edges.
-// Narrow when the frontier has fewer edges than the narrow groups have
-// threads.
+// Direction-optimising BFS rule on estimated frontier edges (frontier size
+// x average degree): top-down -> bottom-up when they exceed the unexplored
+// edges / 14, back when the frontier falls below n / 24; SSSP pulls while
+// the frontier holds over a quarter of the vertices.  Narrow when the
+// frontier's estimated edges are fewer than the narrow groups' threads.
 __device__ inline int heuristic(const FeatIn& a, unsigned valid) {
+  const float deg = a.m / fmaxf(a.n, 1.f);
+  const float fe = a.nin * deg, unv_e = fmaxf(a.n - a.total, 0.f) * deg;
   bool pull = false;
   if (valid & (1u << GDB_ARM_PULL)) {
     if (a.algo == GDB_TASK_BFS)
-      pull = arm_pull(a.prev_arm) ? a.nin >= a.n / 24.f : a.fe > a.unv_e / 14.f;
+      pull = arm_pull(a.prev_arm) ? a.nin >= a.n / 24.f : fe > unv_e / 14.f;
     else
-      pull = a.fe > a.m * 0.25f;
+      pull = a.nin > a.n * 0.25f;
   }
   if (pull) return GDB_ARM_PULL;
-  if ((valid & (1u << GDB_ARM_PUSH_NARROW)) && a.fe < a.narrow_threads) return GDB_ARM_PUSH_NARROW;
+  if ((valid & (1u << GDB_ARM_PUSH_NARROW)) && fe < a.narrow_threads) return GDB_ARM_PUSH_NARROW;
   return GDB_ARM_PUSH;
 }
 
@@ -146,8 +163,13 @@ __device__ inline int decide(const Cfg& c, const unsigned short* lw, const FeatI
       float y[kOut];
       mlp_wave(lw, xi, y);
       int best = GDB_ARM_PUSH;
+      float by = y[0];
+#pragma unroll
       for (int a2 = 1; a2 < kOut; ++a2)
-        if ((valid & (1u << a2)) && y[a2] < y[best]) best = a2;
+        if ((valid & (1u << a2)) && y[a2] < by) {
+          best = a2;
+          by = y[a2];
+        }
       return best;
     }
     default: return GDB_ARM_PUSH;

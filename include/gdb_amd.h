@@ -225,7 +225,11 @@ int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups, 
  * publishes the arm with the barrier release.  MLP mode evaluates a
  * 16-64-64-4 network (bf16 weights, fp32 accumulation) whose outputs predict
  * log2 of the iteration's time in microseconds per arm, and takes the valid
- * arm with the smallest prediction. */
+ * arm with the smallest prediction; it reruns the network only when the
+ * frontier has moved by over 25% (or after 32 iterations).  The choosing
+ * modes offer pull arms to BFS only (a BFS iteration makes the same progress
+ * under every arm, an SSSP pull iteration does not); GDB_SEL_FIXED may pull
+ * SSSP too. */
 #define GDB_SEL_NONE 0       /* PUSH on the whole team every iteration (the plain schedule) */
 #define GDB_SEL_FIXED 1      /* fixed_arm every iteration (when valid) */
 #define GDB_SEL_HEURISTIC 2  /* direction-optimising rule (alpha 14, beta 24) + narrow when the

@@ -156,3 +156,58 @@ def test_mlp_training_reduces_loss(ctx):
     assert l1 < 0.1 * l0 and l1 < 0.5
     # The trained bf16 weights are the rounding of the fp32 master copy.
     assert (w.to_numpy() == gdb.bf16_round(p)).all()
+
+
+def solve_one(ctx, gr, dg, algo, sel, team=64):
+    out = gdb.DeviceBuffer.empty(gr.n, np.uint32)
+    t = gdb.make_task(algo, dg, out, 0, delta=512 if algo == gdb.TASK_SSSP else 0, transpose=dg)
+    best = None
+    for _ in range(3):
+        _, st = gdb.batch_solve(ctx, [t], team, gdb.Opts.make(wgs=256, groups=team), selector=sel)
+        best = st.kernel_ms if best is None else min(best, st.kernel_ms)
+    return out.to_numpy(), best, list(t.arm_iters)
+
+
+def test_trained_selector_gains_over_plain_schedule(ctx):
+    """End to end: traces logged by the engine (random arms) on training
+    graphs -> Adam on the device -> the MLP picks arms on unseen graphs.
+    Outputs stay bit-exact; the learned schedule is faster than the plain
+    push schedule (the method without the selector) on unseen R-MAT graphs,
+    where bottom-up iterations pay off."""
+    cap = 200000
+    trace = gdb.DeviceBuffer.from_numpy(np.zeros(cap * gdb.TRACE_STRIDE, np.float32))
+    cnt = gdb.DeviceBuffer.from_numpy(np.zeros(1, np.int32))
+    runs = 0
+    for g in (gdb.gen_rmat(13, 16, seed=21, wlo=1, whi=255), gdb.gen_rmat(15, 16, seed=22, wlo=1, whi=255),
+              gdb.gen_grid(160, 160, 0.05, seed=23, wlo=1, whi=255)):
+        dg = gdb.DeviceCSR(g)
+        for rep in range(3):
+            for algo in (gdb.TASK_BFS, gdb.TASK_SSSP):
+                sel = gdb.make_selector(gdb.SEL_RANDOM, narrow_groups=2, seed=100 + runs, trace=trace,
+                                        trace_count=cnt)
+                solve_one(ctx, g, dg, algo, sel)
+                runs += 1
+    n = min(int(cnt.to_numpy()[0]), cap)
+    rec = trace.to_numpy().reshape(cap, gdb.TRACE_STRIDE)[:n]
+    rec = np.ascontiguousarray(rec[rec[:, 17] > 0])
+    drec = gdb.DeviceBuffer.from_numpy(rec)
+    p, w, loss = gdb.mlp_train(ctx, gdb.mlp_init(3), drec, len(rec), steps=2000, batch=256, lr=2e-3, seed=3)
+    print("records", len(rec), "loss", loss)
+    gains = {}
+    for name, g in (("rmat16", gdb.gen_rmat(16, 16, seed=61, wlo=1, whi=255)),
+                    ("rmat17", gdb.gen_rmat(17, 16, seed=62, wlo=1, whi=255))):
+        dg = gdb.DeviceCSR(g)
+        for algo in (gdb.TASK_BFS, gdb.TASK_SSSP):
+            ref, t_push, _ = solve_one(ctx, g, dg, algo, None)
+            got, t_mlp, arms = solve_one(ctx, g, dg, algo, gdb.make_selector(gdb.SEL_MLP, narrow_groups=2,
+                                                                            weights=w))
+            assert (got == ref).all()
+            gains[(name, algo)] = (t_push, t_mlp, arms)
+    print(gains)
+    bfs_push = sum(v[0] for k, v in gains.items() if k[1] == gdb.TASK_BFS)
+    bfs_mlp = sum(v[1] for k, v in gains.items() if k[1] == gdb.TASK_BFS)
+    tot_push = sum(v[0] for v in gains.values())
+    tot_mlp = sum(v[1] for v in gains.values())
+    print(f"BFS gain {bfs_push / bfs_mlp:.2f}x, all {tot_push / tot_mlp:.2f}x")
+    assert bfs_mlp < 0.7 * bfs_push
+    assert tot_mlp < tot_push
