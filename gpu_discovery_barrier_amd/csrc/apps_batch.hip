@@ -202,9 +202,9 @@ __device__ int sel_release(Team& T, const int* tail, const SelRun& R, const DevT
       c.trace[(size_t)pending * gdbsel::kStride + gdbsel::kIn + 1] = (float)(t - t_rel) * 0.01f;
   }
   if (nin == 0) {
-    // A far-pile split (or the end): keep the team's width, nothing to learn.
-    arm = gdbsel::arm_narrow(S.arm) ? GDB_ARM_PUSH_NARROW : GDB_ARM_PUSH;
-    if (!(R.valid & (1u << arm))) arm = GDB_ARM_PUSH;
+    // A far-pile split (or the end): the whole team splits the pile, which
+    // can hold most of the graph; nothing to learn.
+    arm = GDB_ARM_PUSH;
   } else {
     gdbsel::FeatIn a;
     a.n = (float)k.n;
@@ -249,76 +249,75 @@ __device__ int sel_release(Team& T, const int* tail, const SelRun& R, const DevT
 
 // Team barrier over the `members` active groups that also picks the next
 // iteration's arm (in the group that completes it) and, when a narrow team
-// widens again, wakes the parked groups.  Returns the payload.
+// widens again, wakes the parked groups.  Returns the payload.  Only wave 0
+// takes part (lane 0 arrives or waits, the whole wave runs the selector), so
+// the group pays the same two workgroup barriers as the plain team barrier.
 __device__ int team_barrier_sel(Team& T, int members, const int* tail, const SelRun& R,
                                 const DevTask& k, const IterState& S) {
-  __shared__ int s_last, s_val, s_gen;
+  __shared__ int s_val;
   gdb::drain_stores();
   __syncthreads();
-  const bool cross = members > 1;
-  unsigned long long* word = (unsigned long long*)T.word(TW_GEN);
-  if (threadIdx.x == 0) {
-    int g = *T.gen_cache;
-    if (g == GDB_GEN_UNKNOWN && T.G > 1)
-      g = (int)(unsigned)__hip_atomic_load(word, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    int last = 1;
-    if (cross) {
-      const int old = gdb::fadd(T.word(TW_COUNT), 1);
-      last = old == members - 1;
-      if (last) gdb::st_rlx(T.word(TW_COUNT), 0);
-    }
-    s_gen = g;
-    s_last = last;
-  }
-  __syncthreads();
-  if (s_last) {
-    if (threadIdx.x < 64) {
-      const int v = sel_release(T, tail, R, k, S);
-      if (threadIdx.x == 0) s_val = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int val = s_val;
-      const int new_arm = (int)((unsigned)val >> kNinBits);
-      const bool wake = members < T.G && !gdbsel::arm_narrow(new_arm);
-      const int ng = cross ? s_gen + 1 : s_gen;
-      if (wake) {
-        int* rs = T.word(TW_RESUME);
-        gdb::st_rlx(rs + RS_DONE, 0);
-        gdb::st_rlx(rs + RS_IT, S.it + 1);
-        gdb::st_rlx(rs + RS_PAYLOAD, val);
-        gdb::st_rlx(rs + RS_FGEN, S.fgen);
-        gdb::st_rlx(rs + RS_THRESH, (int)S.thresh);
-        gdb::st_rlx(rs + RS_BARGEN, ng);
-        gdb::st_rlx(rs + RS_TOTAL, sat_add(S.total, val & kNinMask));
-        const bool ev = R.cfg.mode == GDB_SEL_MLP && R.lw && (val & kNinMask) > 0 && !mlp_reuse(S, val & kNinMask);
-        gdb::st_rlx(rs + RS_EVAL_NIN, ev ? (val & kNinMask) : S.eval_nin);
-        gdb::st_rlx(rs + RS_EVAL_IT, ev ? S.it + 1 : S.eval_it);
-      }
+  if (threadIdx.x < 64) {
+    const int lane = (int)threadIdx.x;
+    const bool cross = members > 1;
+    unsigned long long* word = (unsigned long long*)T.word(TW_GEN);
+    int g = 0, last = 1;
+    if (lane == 0) {
+      g = *T.gen_cache;
+      if (g == GDB_GEN_UNKNOWN && T.G > 1)
+        g = (int)(unsigned)__hip_atomic_load(word, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
       if (cross) {
-        const unsigned long long w = ((unsigned long long)(unsigned)val << 32) | (unsigned)ng;
-        __hip_atomic_store(word, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const int old = gdb::fadd(T.word(TW_COUNT), 1);
+        last = old == members - 1;
+        if (last) gdb::st_rlx(T.word(TW_COUNT), 0);
       }
-      if (wake) gdb::st_rel(T.word(TW_PARK), S.wakes + 1);
-      *T.gen_cache = ng;
     }
-  } else if (threadIdx.x == 0) {
-    const int g = s_gen;
-    unsigned long long w = 0;
-    const bool ok = gdb::wait_while(
-        T.ctx,
-        [&] {
-          w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          return (int)(unsigned)w == g;
-        },
-        GDB_ERR_BARRIER_TIMEOUT);
-    gdb::fence_acq();
-    if (!ok) T.l->aborted = 1;
-    s_val = (int)(w >> 32);
-    *T.gen_cache = g + 1;
+    last = __shfl(last, 0, 64);
+    if (last) {
+      const int val = sel_release(T, tail, R, k, S);
+      if (lane == 0) {
+        const int new_arm = (int)((unsigned)val >> kNinBits);
+        const bool wake = members < T.G && !gdbsel::arm_narrow(new_arm);
+        const int ng = cross ? g + 1 : g;
+        if (wake) {
+          const int nn = val & kNinMask;
+          const bool ev = R.cfg.mode == GDB_SEL_MLP && R.lw && nn > 0 && !mlp_reuse(S, nn);
+          int* rs = T.word(TW_RESUME);
+          gdb::st_rlx(rs + RS_DONE, 0);
+          gdb::st_rlx(rs + RS_IT, S.it + 1);
+          gdb::st_rlx(rs + RS_PAYLOAD, val);
+          gdb::st_rlx(rs + RS_FGEN, S.fgen);
+          gdb::st_rlx(rs + RS_THRESH, (int)S.thresh);
+          gdb::st_rlx(rs + RS_BARGEN, ng);
+          gdb::st_rlx(rs + RS_TOTAL, sat_add(S.total, nn));
+          gdb::st_rlx(rs + RS_EVAL_NIN, ev ? nn : S.eval_nin);
+          gdb::st_rlx(rs + RS_EVAL_IT, ev ? S.it + 1 : S.eval_it);
+        }
+        if (cross) {
+          const unsigned long long w = ((unsigned long long)(unsigned)val << 32) | (unsigned)ng;
+          __hip_atomic_store(word, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (wake) gdb::st_rel(T.word(TW_PARK), S.wakes + 1);
+        *T.gen_cache = ng;
+        s_val = val;
+      }
+    } else if (lane == 0) {
+      unsigned long long w = 0;
+      const bool ok = gdb::wait_while(
+          T.ctx,
+          [&] {
+            w = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return (int)(unsigned)w == g;
+          },
+          GDB_ERR_BARRIER_TIMEOUT);
+      gdb::fence_acq();
+      if (!ok) T.l->aborted = 1;
+      s_val = (int)(w >> 32);
+      *T.gen_cache = g + 1;
+    }
   }
   __syncthreads();
-  return s_val;
+  return __builtin_amdgcn_readfirstlane(s_val);
 }
 
 // A parked group waits until the team's wake count passes `wakes`, then

@@ -120,8 +120,9 @@ __device__ __forceinline__ bool wait_while(gdb_kernel_ctx_t* ctx, Pred pred, int
 }
 
 // Bounded wait for long idle periods (a group parked while the rest of its
-// team works): after 32 quick polls it polls every ~8 us, so that many
-// parked groups do not load the memory system the active groups use.
+// team works): after 32 quick polls it polls about once a microsecond, so
+// that many parked groups do not load the memory system the active groups
+// use, while a wake-up still takes about a microsecond.
 template <typename Pred>
 __device__ __forceinline__ bool wait_while_idle(gdb_kernel_ctx_t* ctx, Pred pred, int err_code) {
   if (!pred()) return true;
@@ -133,11 +134,8 @@ __device__ __forceinline__ bool wait_while_idle(gdb_kernel_ctx_t* ctx, Pred pred
   const unsigned long long t0 = now();
   unsigned it = 0;
   while (pred()) {
-    if (it < 32) {
-      __builtin_amdgcn_s_sleep(2);
-    } else {
-      for (int k = 0; k < 8; ++k) __builtin_amdgcn_s_sleep(127);
-    }
+    if (it < 32) __builtin_amdgcn_s_sleep(2);
+    else __builtin_amdgcn_s_sleep(32);  // 32 x 64 cycles
     if ((++it & 7u) == 0) {
       if (ld_rlx(&ctx->error.v) != GDB_OK) return false;
       if (now() - t0 > limit) {

@@ -18,6 +18,7 @@ constexpr int kW1 = 0, kB1 = kW1 + kIn * kHid, kW2 = kB1 + kHid, kB2 = kW2 + kHi
               kW3 = kB2 + kHid, kB3 = kW3 + kHid * kOut, kParams = kB3 + kOut;
 static_assert(kParams == GDB_MLP_PARAMS, "parameter count");
 constexpr int kStride = GDB_TRACE_STRIDE;
+constexpr float kSwitchMargin = 0.15f;  // log2 units
 
 struct Cfg {
   int mode, fixed_arm, narrow_groups;
@@ -45,7 +46,9 @@ __device__ __forceinline__ bool arm_narrow(int a) { return a >= GDB_ARM_PUSH_NAR
 // counts every group of the team already holds: the next frontier's size,
 // the running sum of frontier sizes (for BFS, the vertices reached), the
 // graph's size and the team's shape; frontier edges are estimated with the
-// average degree.
+// average degree.  They leave out the previous arm: the network predicts
+// the steady-state cost of each arm (it is trained on iterations that kept
+// the arm of the one before).
 struct FeatIn {
   float n, m;           // graph size
   float nin;            // next frontier size
@@ -73,8 +76,8 @@ __device__ __forceinline__ float feature(const FeatIn& a, int i) {
     case 8: return unv / fmaxf(a.n, 1.f);
     case 9: return (float)a.it / ((float)a.it + 64.f);
     case 10: return (float)a.algo;
-    case 11: return arm_pull(a.prev_arm) ? 1.f : 0.f;
-    case 12: return arm_narrow(a.prev_arm) ? 1.f : 0.f;
+    case 11: return lg(a.nin * deg / fmaxf(a.team_threads, 1.f) * 64.f);
+    case 12: return lg(a.nin * deg / fmaxf(a.narrow_threads, 1.f) * 64.f);
     case 13: return lg(a.prev_nin);
     case 14: return lg(a.team_threads);
     default: return lg(a.narrow_threads);
@@ -148,8 +151,14 @@ __device__ inline int decide(const Cfg& c, const unsigned short* lw, const FeatI
       return (c.fixed_arm >= 0 && c.fixed_arm < kOut && (valid & (1u << c.fixed_arm))) ? c.fixed_arm
                                                                                      : GDB_ARM_PUSH;
     case GDB_SEL_RANDOM: {
+      // Sticky random arms: a new draw one iteration in eight, so most
+      // records are taken in steady state (the same arm as the iteration
+      // before, no wake-up of parked groups in the cost).
+      const unsigned h = hash32(c.seed * 0x9e3779b9u ^ salt);
+      if ((h & 7u) != 0 && a.prev_arm >= 0 && a.prev_arm < kOut && (valid & (1u << a.prev_arm)))
+        return a.prev_arm;
       const int cnt = __popc(valid);
-      int pick = (int)(hash32(c.seed * 0x9e3779b9u ^ salt) % (unsigned)cnt);
+      int pick = (int)((h >> 3) % (unsigned)cnt);
       for (int a2 = 0; a2 < kOut; ++a2)
         if (valid & (1u << a2)) {
           if (pick == 0) return a2;
@@ -162,11 +171,19 @@ __device__ inline int decide(const Cfg& c, const unsigned short* lw, const FeatI
       if (!lw) return heuristic(a, valid);
       float y[kOut];
       mlp_wave(lw, xi, y);
-      int best = GDB_ARM_PUSH;
-      float by = y[0];
+      // Smallest predicted cost; leaving the current arm needs a predicted
+      // gain of over 10% (0.15 in log2), so that model noise does not
+      // flip the schedule back and forth.
+      const int cur = (a.prev_arm >= 0 && a.prev_arm < kOut && (valid & (1u << a.prev_arm))) ? a.prev_arm
+                                                                                            : GDB_ARM_PUSH;
+      int best = cur;
+      float by = 0.f;
 #pragma unroll
-      for (int a2 = 1; a2 < kOut; ++a2)
-        if ((valid & (1u << a2)) && y[a2] < by) {
+      for (int a2 = 0; a2 < kOut; ++a2)
+        if (a2 == cur) by = y[a2] - kSwitchMargin;
+#pragma unroll
+      for (int a2 = 0; a2 < kOut; ++a2)
+        if (a2 != cur && (valid & (1u << a2)) && y[a2] < by) {
           best = a2;
           by = y[a2];
         }

@@ -79,10 +79,38 @@ def solve(ctx, gr: Graph, algo: int, source: int, sel: gdb.Selector | None):
 NARROW = 2
 
 
+def steady_records(rec: np.ndarray) -> np.ndarray:
+    """Records with a measured cost whose iteration kept the arm of the
+    iteration before (same task, consecutive iterations): their cost holds
+    no wake-up of parked groups or other switching cost.  Records of one
+    solve are contiguous and in iteration order."""
+    rec = np.asarray(rec)
+    if len(rec) < 2:
+        return rec[:0]
+    prev, cur = rec[:-1], rec[1:]
+    keep = ((cur[:, 17] > 0) & (prev[:, 18] == cur[:, 18]) & (prev[:, 19] == cur[:, 19] - 1)
+            & (prev[:, 16] == cur[:, 16]))
+    return np.ascontiguousarray(cur[keep])
+
+
+def balanced(parts: list[np.ndarray], max_factor: int = 64) -> np.ndarray:
+    """Concatenates per-graph record sets, repeating the smaller ones (up to
+    max_factor times) so that each graph weighs about the same in training:
+    a high-diameter grid logs thousands of iterations, an R-MAT graph ten."""
+    parts = [p for p in parts if len(p)]
+    if not parts:
+        return np.zeros((0, gdb.TRACE_STRIDE), np.float32)
+    big = max(len(p) for p in parts)
+    out = [np.tile(p, (min(max_factor, max(1, big // len(p))), 1)) for p in parts]
+    return np.ascontiguousarray(np.concatenate(out))
+
+
 def collect(ctx, names, sources_per_graph, cap, seed):
+    """Random-arm runs on each graph; returns the raw records of each graph."""
     trace = gdb.DeviceBuffer.from_numpy(np.zeros(cap * gdb.TRACE_STRIDE, np.float32))
     cnt = gdb.DeviceBuffer.from_numpy(np.zeros(1, np.int32))
     runs = 0
+    bounds = [0]
     for name in names:
         gr = make(name)
         rng = np.random.default_rng(seed)
@@ -95,10 +123,10 @@ def collect(ctx, names, sources_per_graph, cap, seed):
                 t, st = solve(ctx, gr, algo, s, sel)
                 runs += 1
         log(f"  traces so far: {int(cnt.to_numpy()[0])}")
+        bounds.append(min(int(cnt.to_numpy()[0]), cap))
         del gr
-    n = min(int(cnt.to_numpy()[0]), cap)
-    rec = trace.to_numpy().reshape(cap, gdb.TRACE_STRIDE)[:n].copy()
-    return rec[rec[:, 17] > 0]
+    rec = trace.to_numpy().reshape(cap, gdb.TRACE_STRIDE)
+    return [rec[bounds[i]:bounds[i + 1]].copy() for i in range(len(names))]
 
 
 def evaluate(ctx, names, weights: gdb.DeviceBuffer, reps: int, fixed_arms: bool = False):
@@ -153,15 +181,18 @@ def main() -> int:
         train = ["rmat:14:1", "grid:2048:2", "rmat:22:3"]
         test = ["rmat:20:50", "rmat:21:51", "rmat:22:52", "rmat:23:53", "grid:1024:7", "grid:2048:8",
                 "grid:4096:9"]
-        nsrc, reps = 3, 3
+        nsrc, reps = 6, 3
     ctx = gdb.Context()
     report = {"train_graphs": train, "test_graphs": test, "narrow_groups": NARROW}
     if not args.eval_only:
         log("collecting traces ...")
-        rec = collect(ctx, train, nsrc, 400_000, seed=5)
-        log(f"{len(rec)} records; arms {np.bincount(rec[:, 16].astype(int), minlength=4).tolist()}")
+        raw = collect(ctx, train, nsrc, 400_000, seed=5)
         if args.traces:
-            np.save(args.traces, rec)
+            np.save(args.traces, np.concatenate(raw))
+        steady = [steady_records(r) for r in raw]
+        rec = balanced(steady)
+        log(f"{sum(len(r) for r in raw)} records, steady per graph {[len(r) for r in steady]}, "
+            f"{len(rec)} after balancing; arms {np.bincount(rec[:, 16].astype(int), minlength=4).tolist()}")
         drec = gdb.DeviceBuffer.from_numpy(rec)
         p0 = gdb.mlp_init(5)
         _, l0, _ = gdb.mlp_grad(ctx, gdb.DeviceBuffer.from_numpy(gdb.bf16_round(p0)), drec, len(rec))
@@ -169,7 +200,8 @@ def main() -> int:
         p, w, l1 = gdb.mlp_train(ctx, p0, drec, len(rec), steps=args.steps, batch=512, lr=2e-3, seed=5)
         log(f"trained: loss {l0:.3f} -> {l1:.4f} in {time.time() - t0:.1f}s")
         np.save(args.weights, gdb.bf16_round(p))
-        report.update({"records": int(len(rec)), "loss_initial": l0, "loss_final": l1, "steps": args.steps})
+        report.update({"records": int(sum(len(r) for r in raw)), "steady_records": [len(r) for r in steady],
+                       "loss_initial": l0, "loss_final": l1, "steps": args.steps})
     wb = np.load(args.weights)
     weights = gdb.DeviceBuffer.from_numpy(wb.astype(np.uint16))
     log("evaluating ...")

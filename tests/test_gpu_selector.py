@@ -10,6 +10,7 @@ import pytest
 
 import gpu_discovery_barrier_amd as gdb
 import oracle
+from gpu_discovery_barrier_amd.selector_train import balanced, steady_records
 
 pytestmark = pytest.mark.gpu
 
@@ -80,14 +81,15 @@ def test_fixed_arm_parity(ctx, name, arm):
 @pytest.mark.parametrize("team,narrow", [(1, 1), (4, 1), (16, 3), (64, 1)])
 @pytest.mark.parametrize("mode", [gdb.SEL_RANDOM, gdb.SEL_HEURISTIC])
 def test_switching_arms_parity(ctx, mode, team, narrow):
-    # Random arms switch schedule (and team width) nearly every iteration:
-    # parked groups are woken and parked again many times.
+    # Random arms redraw one iteration in eight: on the grid's ~190 levels
+    # the schedule (and team width) switches dozens of times, so parked
+    # groups are woken and parked again many times.
     for seed, name in enumerate(["rmat12", "grid96", "rmat11_directed"]):
         g = GRAPHS[name]
         sel = gdb.make_selector(mode, narrow_groups=narrow, seed=seed + 11)
         outs, tasks, st = run(ctx, g, sel, team=team, delta=500)
         check(g, (gdb.TASK_BFS, gdb.TASK_SSSP), outs)
-        if mode == gdb.SEL_RANDOM and team > narrow:
+        if mode == gdb.SEL_RANDOM and team > narrow and name == "grid96":
             assert sum(1 for a in range(4) if tasks[0].arm_iters[a] > 0) >= 3
 
 
@@ -178,6 +180,7 @@ def test_trained_selector_gains_over_plain_schedule(ctx):
     trace = gdb.DeviceBuffer.from_numpy(np.zeros(cap * gdb.TRACE_STRIDE, np.float32))
     cnt = gdb.DeviceBuffer.from_numpy(np.zeros(1, np.int32))
     runs = 0
+    bounds = [0]
     for g in (gdb.gen_rmat(13, 16, seed=21, wlo=1, whi=255), gdb.gen_rmat(15, 16, seed=22, wlo=1, whi=255),
               gdb.gen_grid(160, 160, 0.05, seed=23, wlo=1, whi=255)):
         dg = gdb.DeviceCSR(g)
@@ -187,9 +190,9 @@ def test_trained_selector_gains_over_plain_schedule(ctx):
                                         trace_count=cnt)
                 solve_one(ctx, g, dg, algo, sel)
                 runs += 1
-    n = min(int(cnt.to_numpy()[0]), cap)
-    rec = trace.to_numpy().reshape(cap, gdb.TRACE_STRIDE)[:n]
-    rec = np.ascontiguousarray(rec[rec[:, 17] > 0])
+        bounds.append(min(int(cnt.to_numpy()[0]), cap))
+    raw = trace.to_numpy().reshape(cap, gdb.TRACE_STRIDE)
+    rec = balanced([steady_records(raw[bounds[i]:bounds[i + 1]]) for i in range(3)])
     drec = gdb.DeviceBuffer.from_numpy(rec)
     p, w, loss = gdb.mlp_train(ctx, gdb.mlp_init(3), drec, len(rec), steps=2000, batch=256, lr=2e-3, seed=3)
     print("records", len(rec), "loss", loss)
