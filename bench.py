@@ -11,7 +11,8 @@ with dverifysolution.  Also reported: us per barrier vs us per relaunch at the
 tuned launch shape, the roofline of the dominant kernel, and a CPU baseline
 (the oracle's single-thread restatement on the same graph).
 
-Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--config c2]
+(--config c1 / c3 / c4 measure the other configurations the same way.)
 """
 from __future__ import annotations
 
@@ -28,6 +29,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# wgs, groups, near/far band, local hops, selector mode (0 none, 4 MLP)
+TUNED_C2 = (256, 64, 2048, 16, 0)
 GRID_SIDE = 2048
 
 
@@ -40,12 +43,17 @@ def reached_edges(g, dist, inf) -> int:
     return int(deg[dist != inf].sum())
 
 
-def traffic_from_profiles(kernel_prefix: str):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if any."""
-    path = ROOT / "profiles" / "wl_tuned_c2_traffic.json"
+def traffic_from_profiles(shape: tuple):
+    """HBM bytes per launch of the batched kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the same
+    counters' reading for a copy kernel of known size), when it was taken at
+    this launch shape; else None."""
+    path = ROOT / "profiles" / "batch_grid2048_traffic.json"
     try:
         d = json.loads(path.read_text())
-        return d.get(kernel_prefix)
+        if tuple(d["shape"]) != tuple(shape):
+            return None
+        return d["hbm_bytes_per_launch"]
     except Exception:
         return None
 
@@ -69,18 +77,272 @@ def cpu_baseline(g, seconds_cap: float = 30.0) -> dict:
     }
 
 
+def line(metric, value, unit, args, ms_per_step, config, dtype, extra) -> dict:
+    out = {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def timed(fn, warmup, steps):
+    """Runs fn() -> kernel ms; returns (median kernel ms, median wall ms, last result)."""
+    for _ in range(warmup):
+        fn()
+    ks, ws, res = [], [], None
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        res = fn()
+        ws.append((time.perf_counter() - t0) * 1e3)
+        ks.append(res[0])
+    return float(np.median(ks)), float(np.median(ws)), res
+
+
+def bench_c1(args) -> int:
+    """Pannotia max-rule colouring, R-MAT scale 14 (edge factor 16, seed 1),
+    priorities a seeded permutation.  Persistent engine (autotuned launch
+    shape, one barrier per round) vs the plain persistent engine (the
+    reference kernel's structure, default launch).  GTEPS = m / time."""
+    import gpu_discovery_barrier_amd as gdb
+    import oracle
+
+    g = gdb.gen_rmat(14, 16, seed=1)
+    dg = gdb.DeviceCSR(g, weights=False)
+    nv_h = (gdb.permutation(g.n, 1).astype(np.float64) / (g.n + 1)).astype(np.float32)
+    nv = gdb.DeviceBuffer.from_numpy(nv_h)
+    ctx = gdb.Context()
+    out = gdb.DeviceBuffer.empty(g.n, np.int32)
+
+    def run(variant, wgs, groups):
+        c, st = gdb.color(ctx, dg, nv, gdb.Opts.make(wgs=wgs, groups=groups, variant=variant), out=out)
+        return st.kernel_ms, c, st
+
+    trials = []
+    for w in (64, 128, 256, 512, 1024):
+        for gr in (4, 8, 16, 32, 64, 128):
+            if gr <= gdb.max_resident_groups(w, 0):
+                trials.append((min(run(gdb.VARIANT_TUNED, w, gr)[0] for _ in range(3)), w, gr))
+    trials.sort()
+    _, wgs, groups = trials[0]
+    t_ms, wall, (_, c_t, st_t) = timed(lambda: run(gdb.VARIANT_TUNED, wgs, groups), args.warmup, args.steps)
+    p_ms, _, (_, c_p, st_p) = timed(lambda: run(gdb.VARIANT_PLAIN, 256, 0), 1, 3)
+    want, rounds = oracle.color_ref(g.row, g.col, nv_h)
+    exact = bool((c_t == want).all() and (c_p == want).all())
+    t0 = time.perf_counter()
+    oracle.color_ref(g.row, g.col, nv_h)
+    t_cpu = time.perf_counter() - t0
+    gteps = g.m / (t_ms / 1e3) / 1e9
+    o = line("GTEPS on 1 MI355X (persistent autotuned vs plain, bit-exact)", gteps, "GTEPS", args, wall,
+             {"workload": f"Pannotia max-rule colour, R-MAT scale 14 ef 16 seed 1 (n={g.n}, m={g.m}), "
+                          "priorities = seeded permutation",
+              "wgs": wgs, "groups": groups, "participating": st_t.participating, "rounds": st_t.rounds},
+             "f32", {
+                 "kernel_ms": round(t_ms, 4), "plain": {"gteps": round(g.m / (p_ms / 1e3) / 1e9, 4),
+                                                        "ms": round(p_ms, 4), "participating": st_p.participating,
+                                                        "barriers": st_p.barriers},
+                 "speedup_vs_plain": round(p_ms / t_ms, 3), "bit_exact_vs_plain": exact,
+                 "oracle_rounds": rounds,
+                 "cpu_baseline": {"value": round(g.m / t_cpu / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+                                  "sample": f"one full colouring by the oracle restatement ({t_cpu:.3f}s)"}})
+    ctx.close()
+    print(json.dumps(o), flush=True)
+    return 0 if exact else 3
+
+
+def bench_c3(args) -> int:
+    """Pannotia MIS and BC on R-MAT scale 22 (edge factor 16, seed 3).  MIS
+    priorities: a seeded permutation; BC: 64 seeded sources, f32 sigma /
+    delta.  Tuned persistent kernels (autotuned launch) vs the plain
+    persistent kernels.  value = GTEPS of MIS + BC together:
+    (m + 64 m_r) / (t_mis + t_bc)."""
+    import gpu_discovery_barrier_amd as gdb
+    import oracle
+
+    log("generating R-MAT 22 ...")
+    g = gdb.gen_rmat(22, 16, seed=3)
+    dg = gdb.DeviceCSR(g, weights=False)
+    nv_h = (gdb.permutation(g.n, 3).astype(np.float64) / (g.n + 1)).astype(np.float32)
+    nv = gdb.DeviceBuffer.from_numpy(nv_h)
+    deg = np.diff(g.row)
+    rng = np.random.default_rng(3)
+    sources = np.sort(rng.choice(np.flatnonzero(deg > 0), 64, replace=False)).astype(np.int32)
+    ctx = gdb.Context()
+    mis_out = gdb.DeviceBuffer.empty(g.n, np.int32)
+    bc_out = gdb.DeviceBuffer.empty(g.n, np.float32)
+
+    def mis(variant, wgs, groups):
+        s, st = gdb.mis(ctx, dg, nv, gdb.Opts.make(wgs=wgs, groups=groups, variant=variant), out=mis_out)
+        return st.kernel_ms, s, st
+
+    def bc(variant, wgs, groups):
+        b, st = gdb.bc(ctx, dg, dg, sources, gdb.Opts.make(wgs=wgs, groups=groups, variant=variant), out=bc_out)
+        return st.kernel_ms, b, st
+
+    shapes = [(w, gr) for w in (256, 512, 1024) for gr in (256, 512, 1024, 2048)
+              if gr <= gdb.max_resident_groups(w, 0)]
+    mt = sorted((mis(gdb.VARIANT_TUNED, w, gr)[0], w, gr) for w, gr in shapes)[0]
+    bt = sorted((bc(gdb.VARIANT_TUNED, w, gr)[0], w, gr) for w, gr in shapes[::2])[0]
+    log(f"autotune: mis wgs={mt[1]} groups={mt[2]}, bc wgs={bt[1]} groups={bt[2]}")
+    m_ms, wall_m, (_, s_t, st_m) = timed(lambda: mis(gdb.VARIANT_TUNED, mt[1], mt[2]), args.warmup, args.steps)
+    b_ms, wall_b, (_, b_t, st_b) = timed(lambda: bc(gdb.VARIANT_TUNED, bt[1], bt[2]), 1, max(1, args.steps // 2))
+    pm_ms, _, (_, s_p, _) = timed(lambda: mis(gdb.VARIANT_PLAIN, 256, 0), 0, 1)
+    pb_ms, _, (_, b_p, _) = timed(lambda: bc(gdb.VARIANT_PLAIN, 256, 0), 0, 1)
+    # m_r per source: edges of the vertices each BFS reaches.
+    lv = gdb.DeviceBuffer.empty(g.n, np.uint32)
+    mr = 0
+    for s0 in sources:
+        d, _ = gdb.bfs_worklist(ctx, dg, int(s0), gdb.Opts.make(variant=gdb.VARIANT_TUNED, wgs=512, groups=256), out=lv)
+        mr += int(deg[d != gdb.LONESTAR_INF].sum())
+    t0 = time.perf_counter()
+    want = oracle.mis_tb(g.row, g.col, nv_h)
+    t_cpu = time.perf_counter() - t0
+    mis_exact = bool((s_t == want).all() and (s_p == want).all())
+    denom = np.maximum(np.abs(b_p.astype(np.float64)), 1.0)
+    bc_rel = float(np.max(np.abs(b_t.astype(np.float64) - b_p) / denom))
+    total_ms = m_ms + b_ms
+    gteps = (g.m + mr) / (total_ms / 1e3) / 1e9
+    o = line("GTEPS on 1 MI355X (tuned persistent vs plain; MIS bit-exact, BC f32 within 1e-4)", gteps, "GTEPS",
+             args, wall_m + wall_b,
+             {"workload": f"Pannotia MIS + BC (64 sources), R-MAT scale 22 ef 16 seed 3 (n={g.n}, m={g.m})",
+              "mis": {"wgs": mt[1], "groups": mt[2], "rounds": st_m.rounds},
+              "bc": {"wgs": bt[1], "groups": bt[2], "sources": 64}},
+             "f32", {
+                 "mis": {"ms": round(m_ms, 3), "gteps": round(g.m / m_ms / 1e6, 4), "plain_ms": round(pm_ms, 3),
+                         "speedup_vs_plain": round(pm_ms / m_ms, 3), "bit_exact": mis_exact},
+                 "bc": {"ms": round(b_ms, 3), "gteps": round(mr / b_ms / 1e6, 4), "plain_ms": round(pb_ms, 3),
+                        "speedup_vs_plain": round(pb_ms / b_ms, 3), "max_rel_diff_vs_plain": bc_rel, "m_r_sum": mr},
+                 "speedup_vs_plain": round((pm_ms + pb_ms) / total_ms, 3),
+                 "cpu_baseline": {"value": round(g.m / t_cpu / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+                                  "sample": f"greedy sequential MIS of the same graph (oracle, {t_cpu:.2f}s)"}})
+    ctx.close()
+    print(json.dumps(o), flush=True)
+    return 0 if mis_exact and bc_rel < 1e-4 else 3
+
+
+def bench_c4(args) -> int:
+    """Batched: 4096 graphs (2048 G(n,p), n log-uniform in [256, 65536],
+    average degree 8; 2048 grids, side uniform in [16, 256]; weights in
+    [1, 100]; seeds 4 + graph index), each solved with BFS, SSSP and colour:
+    12288 tasks in one persistent launch vs the plain persistent engine one
+    task per launch (timed on every 16th graph; its GTEPS is the sample's).
+    GTEPS = sum of m_r (BFS, SSSP) and m (colour) / time."""
+    import gpu_discovery_barrier_amd as gdb
+    import oracle
+
+    rng = np.random.default_rng(4)
+    log("generating 4096 graphs ...")
+    gs = []
+    for i in range(2048):
+        n = int(np.floor(np.exp(rng.uniform(np.log(256), np.log(65536)))))
+        gs.append(gdb.gen_gnp(n, 8.0, seed=4 + i, wlo=1, whi=100))
+    for i in range(2048):
+        sd = int(rng.integers(16, 257))
+        gs.append(gdb.gen_grid(sd, sd, 0.0, seed=4 + 2048 + i, wlo=1, whi=100))
+    log("uploading ...")
+    dgs = [gdb.DeviceCSR(g) for g in gs]
+    nvs = [gdb.DeviceBuffer.from_numpy(gdb.pannotia_values(g.n)) for g in gs]
+    outs = [[gdb.DeviceBuffer.empty(g.n, np.uint32), gdb.DeviceBuffer.empty(g.n, np.uint32),
+             gdb.DeviceBuffer.empty(g.n, np.int32)] for g in gs]
+    ctx = gdb.Context()
+    N = sum(g.n for g in gs)
+    M = sum(g.m for g in gs)
+
+    def tasks(delta):
+        ts = []
+        for dg, nv, o in zip(dgs, nvs, outs):
+            ts.append(gdb.make_task(gdb.TASK_BFS, dg, o[0], 0))
+            ts.append(gdb.make_task(gdb.TASK_SSSP, dg, o[1], 0, delta=delta))
+            ts.append(gdb.make_task(gdb.TASK_COLOR, dg, o[2], 0, node_value=nv))
+        return ts
+
+    def run(wgs, groups, team, delta):
+        ts = tasks(delta)
+        _, st = gdb.batch_solve(ctx, ts, team, gdb.Opts.make(wgs=wgs, groups=groups))
+        return st.kernel_ms, st
+
+    trials = []
+    for w in (128, 256, 512):
+        for gr in (512, 1024, 2048):
+            if gr > gdb.max_resident_groups(w, 0):
+                continue
+            for dl in (0, 200):
+                trials.append((run(w, gr, 1, dl)[0], w, gr, dl))
+    trials.sort()
+    _, wgs, groups, delta = trials[0]
+    log(f"autotune: wgs={wgs} groups={groups} delta={delta} ({trials[0][0]:.1f} ms)")
+    k_ms, wall, (_, st) = timed(lambda: run(wgs, groups, 1, delta), args.warmup, args.steps)
+    # Reached edges (BFS and SSSP reach the same vertices) and parity on a sample.
+    deg_r = []
+    exact = True
+    plain_ms, plain_edges = 0.0, 0
+    for i, (g, dg, nv, o) in enumerate(zip(gs, dgs, nvs, outs)):
+        lv = o[0].to_numpy()
+        mr = int(np.diff(g.row)[lv != gdb.LONESTAR_INF].sum())
+        deg_r.append(mr)
+        if i % 16 == 0:
+            want_b, _ = oracle.bfs_worklist_ref(g.row, g.col, 0)
+            want_s = oracle.dijkstra_tb(g.row, g.col, g.w, 0)
+            want_c, _ = oracle.color_ref(g.row, g.col, gdb.pannotia_values(g.n))
+            exact &= bool((lv == want_b).all() and (o[1].to_numpy().astype(np.int64) == want_s).all()
+                          and (o[2].to_numpy() == want_c).all())
+            po = gdb.Opts.make(variant=gdb.VARIANT_PLAIN)
+            _, s1 = gdb.bfs_worklist(ctx, dg, 0, po)
+            _, s2 = gdb.sssp_worklist(ctx, dg, 0, po)
+            _, s3 = gdb.color(ctx, dg, nv, po)
+            plain_ms += s1.kernel_ms + s2.kernel_ms + s3.kernel_ms
+            plain_edges += 2 * mr + g.m
+    work = 2 * sum(deg_r) + M
+    gteps = work / (k_ms / 1e3) / 1e9
+    plain_gteps = plain_edges / (plain_ms / 1e3) / 1e9
+    o = line("GTEPS on 1 MI355X (batched vs plain, bit-exact)", gteps, "GTEPS", args, wall,
+             {"workload": f"4096 graphs (2048 G(n,p) + 2048 grids, n={N}, m={M}) x BFS, SSSP, colour",
+              "wgs": wgs, "groups": groups, "team_groups": 1, "delta": delta, "participating": st.participating},
+             "u32", {"kernel_ms": round(k_ms, 3), "work_edges": work,
+                     "plain": {"gteps": round(plain_gteps, 4), "sample": "every 16th graph, one task per launch",
+                               "ms_sample": round(plain_ms, 3)},
+                     "speedup_vs_plain": round(gteps / plain_gteps, 3), "bit_exact_sample": exact})
+    ctx.close()
+    print(json.dumps(o), flush=True)
+    return 0 if exact else 3
+
+
+def copy_calibration(args) -> int:
+    """Three device-to-device copies of 256 MiB (the runtime's copy kernel):
+    the reference reading for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import gpu_discovery_barrier_amd as gdb
+
+    nbytes = 256 << 20
+    a = gdb.DeviceBuffer(nbytes, np.uint8)
+    b = gdb.DeviceBuffer(nbytes, np.uint8)
+    for _ in range(3):
+        gdb.check(gdb.lib().gdb_memcpy_dtod(b.ptr, a.ptr, nbytes), "gdb_memcpy_dtod")
+    gdb.check(gdb.lib().gdb_device_synchronize(), "gdb_device_synchronize")
+    print(json.dumps({"copy_bytes": nbytes, "copies": 3}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-tune", action="store_true", help="use the default launch shape")
+    ap.add_argument("--tune", action="store_true",
+                    help="c2: rerun the autotuner (launch shape, band, hops, selector) instead of the tuned shape")
+    ap.add_argument("--shape", default="",
+                    help="c2: wgs,groups,delta,hops,selector (skips the autotuner; selector 0 none, 4 MLP)")
+    ap.add_argument("--engine-only", action="store_true",
+                    help="c2: time the batched engine only (profiler runs)")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "copy"],
+                    help="c2 (default): the headline grid BFS + SSSP; c1 colour R-MAT 14; "
+                         "c3 MIS + BC R-MAT 22; c4 4096-graph batch; copy: counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(this build runs a single MI355X)", file=sys.stderr)
         return 2
+    if args.config != "c2":
+        return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "copy": copy_calibration}[args.config](args)
 
     import gpu_discovery_barrier_amd as gdb
     from gpu_discovery_barrier_amd import autotune
@@ -129,8 +391,12 @@ def main() -> int:
     # band, local hops, and whether the learned selector schedules the
     # iterations (it needs hops = 0).
     modes = [gdb.SEL_NONE] + ([gdb.SEL_MLP] if weights is not None else [])
-    if args.no_tune:
-        wgs, groups, delta, hops, mode = 256, 64, 4096, 0, modes[-1]
+    if args.shape:
+        wgs, groups, delta, hops, mode = (int(x) for x in args.shape.split(","))
+    elif not args.tune:
+        # The autotuner's pick on this workload (python bench.py --tune):
+        # 144 shapes within 2% of each other at the top.
+        wgs, groups, delta, hops, mode = TUNED_C2
     else:
         trials = []
         for w in (256, 512, 1024):
@@ -162,6 +428,13 @@ def main() -> int:
     mr = reached_edges(g, db, inf)
     t_step_ms = float(np.median(step_ms))
     gteps = 2 * mr / (t_step_ms / 1e3) / 1e9
+    if args.engine_only:
+        # Only the batched launches (for profiler runs): no comparisons.
+        print(json.dumps({"kernel_ms": round(t_step_ms, 3), "gteps": round(gteps, 4),
+                          "shape": [wgs, groups, delta, hops, mode],
+                          "work_edges": bst.work_edges, "work_items": bst.work_items}), flush=True)
+        ctx.close()
+        return 0
 
     # Single-task tuned kernels (one solve per launch) at their own best shape.
     tb, tsb, tds, tss = solve(gdb.VARIANT_TUNED, 512, 128)
@@ -233,7 +506,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": traffic_from_profiles("batch"),
+            "traffic": traffic_from_profiles((wgs, groups, delta, hops, mode)),
             "kernel": "k_batch<WGS> (BFS + SSSP teams)",
             "algorithmic_bytes": int(alg_bytes),
         },
