@@ -19,10 +19,12 @@
 //                      vertices; rho and delta accumulate with float atomics.
 //   tuned persistent - level queues, one barrier per level: the phase that
 //                      expands level l (claiming level l+1 vertices with a
-//                      compare-and-swap on d) also pulls rho for the level-l
-//                      vertices from their in-edges; the backward pass pulls
-//                      delta from out-edges.  No float atomics, so results do
-//                      not depend on scheduling.
+//                      compare-and-swap on the level) also pulls rho for the
+//                      level-l vertices from their in-edges (one walk when the
+//                      graph is symmetric); the backward pass pulls delta
+//                      from out-edges.  Level and rho (later (1+delta)/rho)
+//                      share an 8-byte word: one gather per edge.  No float
+//                      atomics, so results do not depend on scheduling.
 //   relaunch         - bfs_kernel / backtrack_kernel per level with a host
 //                      loop (bc.cpp:322-419).
 #include "graph_common.hpp"
@@ -113,53 +115,60 @@ __global__ __launch_bounds__(WGS) void k_bc_plain(gdb_kernel_ctx_t* ctx, const i
 
 // Level-queue BC.  q holds the reached vertices of the current source level
 // by level; lvl_off[l] is the start of level l in q (lvl_off has n+2 slots).
+// Per-vertex state of the tuned schedule in one 8-byte word, so that an edge
+// costs one gather: x = level (-1 unreached); y = the bits of sigma in the
+// forward pass, then of coef = (1 + delta) / sigma once the vertex's delta
+// is final in the backward pass (delta itself is only needed there).
+__device__ __forceinline__ int2 ld_state(const int2* ds, int v) {
+  const long long x = *(const long long*)&ds[v];
+  return make_int2((int)(x & 0xffffffffll), (int)(x >> 32));
+}
+
 template <int WGS>
 __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
                                                   const int* __restrict__ col,
                                                   const int* __restrict__ row_t,
-                                                  const int* __restrict__ col_t, int* d, float* sigma,
-                                                  float* delta, float* bc, const int* sources,
-                                                  int nsrc, int* q, int* tail, int n, int* out) {
+                                                  const int* __restrict__ col_t, int2* ds, float* bc,
+                                                  const int* sources, int nsrc, int* q, int* tail, int n,
+                                                  int* out) {
   DISCOVERY_PROTOCOL(ctx);
   const int tid = p_get_global_id(ctx, &local_ctx);
   const int stride = p_get_global_size(ctx, &local_ctx);
+  // A symmetric graph is its own transpose: one walk over a frontier
+  // vertex's edges both sums sigma from level l-1 and claims level l+1.
+  const bool sym = row == row_t && col == col_t;
   int nbar = 0, levels = 0;
   for (int i = tid; i < n; i += stride) {
     bc[i] = 0.f;
-    d[i] = -1;
-    sigma[i] = 0.f;
-    delta[i] = 0.f;
+    ds[i] = make_int2(-1, 0);
   }
+  auto plus = [](float a, float c) { return a + c; };
   int prev_reached = 0;
   for (int si = 0; si < nsrc; ++si) {
     const int s = sources[si];
     // Reset what the previous source touched (its reached set is q[0..)).
     for (int i = tid; i < prev_reached; i += stride) {
       const int v = q[i];
-      if ((unsigned)v >= (unsigned)n) continue;
-      d[v] = -1;
-      sigma[v] = 0.f;
-      delta[v] = 0.f;
+      if ((unsigned)v < (unsigned)n) ds[v] = make_int2(-1, 0);
     }
     if (!discovery_barrier(ctx, &local_ctx)) return;
     ++nbar;
     if (tid == 0) {
-      d[s] = 0;
-      sigma[s] = 1.f;
+      ds[s] = make_int2(0, __float_as_int(1.f));
       q[0] = s;
       gdb::st_rlx(tail, 1);
     }
     if (!discovery_barrier(ctx, &local_ctx)) return;
     ++nbar;
     // Forward: level l occupies q[lb, le).  Expanding level l claims level
-    // l+1 vertices; the same phase pulls sigma for level l (l >= 1) from
-    // in-edges whose source sits on level l-1.
-    // Level boundaries live in registers: every group reads the same tail
-    // after each barrier.
+    // l+1 vertices (compare-and-swap on the level word); the same phase
+    // pulls sigma for level l (l >= 1) from in-edges whose source sits on
+    // level l-1.  Level boundaries live in registers: every group reads the
+    // same tail after each barrier.
     int lb = 0, le = 1, l = 0;
     for (;;) {
       GDB_FOR_UNIFORM(i, valid, tid, stride, le - lb) {
-        int v = -1, b = 0, e = 0, bt = 0, et = 0;
+        int v = -1, b = 0, e = 0;
         if (valid) {
           v = q[lb + i];
           if ((unsigned)v >= (unsigned)n) valid = 0;
@@ -167,33 +176,58 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
         if (valid) {
           b = row[v];
           e = row[v + 1];
-          if (l > 0) {
+        }
+        // Claims w for level l+1 if it is unreached (all lanes call it).
+        auto claim = [&](bool ok, int w, int lvl) {
+          const bool push = ok && lvl == -1 && atomicCAS(&ds[w].x, -1, l + 1) == -1;
+          gdb::wave_push(push, w, q, tail);
+        };
+        float sg = 0.f;
+        if (sym) {
+          sg = gdb::wave_coop_walk(
+              valid != 0, b, e, 0.f,
+              [&](int k, bool ok) -> float {
+                int w = 0;
+                int2 x = make_int2(0, 0);
+                if (ok) {
+                  w = col[k];
+                  ok = (unsigned)w < (unsigned)n;
+                  if (ok) x = ld_state(ds, w);
+                }
+                claim(ok, w, x.x);
+                return (ok && l > 0 && x.x == l - 1) ? __int_as_float(x.y) : 0.f;
+              },
+              plus, kSerialLimit);
+        } else {
+          int bt = 0, et = 0;
+          if (valid && l > 0) {
             bt = row_t[v];
             et = row_t[v + 1];
           }
+          sg = gdb::wave_coop_reduce(
+              valid != 0 && l > 0, bt, et, 0.f,
+              [&](int k) -> float {
+                const int u = col_t[k];
+                if ((unsigned)u >= (unsigned)n) return 0.f;
+                const int2 x = ld_state(ds, u);
+                return x.x == l - 1 ? __int_as_float(x.y) : 0.f;
+              },
+              plus, kSerialLimit);
+          gdb::wave_coop_walk(
+              valid != 0, b, e, 0.f,
+              [&](int k, bool ok) -> float {
+                int w = 0, lvl = 0;
+                if (ok) {
+                  w = col[k];
+                  ok = (unsigned)w < (unsigned)n;
+                  if (ok) lvl = ds[w].x;
+                }
+                claim(ok, w, lvl);
+                return 0.f;
+              },
+              plus, kSerialLimit);
         }
-        if (l > 0) {
-          auto edge = [&](int k) -> float {
-            const int u = col_t[k];
-            return d[u] == l - 1 ? sigma[u] : 0.f;
-          };
-          auto red = [](float a, float c) { return a + c; };
-          const float sg =
-              gdb::wave_coop_reduce(valid != 0, bt, et, 0.f, edge, red, kSerialLimit);
-          if (valid) sigma[v] = sg;
-        }
-        gdb::wave_coop_foreach(
-            valid != 0, b, e, 0u,
-            [&](int k, bool ok, unsigned) {
-              bool push = false;
-              int w = 0;
-              if (ok) {
-                w = col[k];
-                if (d[w] == -1) push = atomicCAS(&d[w], -1, l + 1) == -1;
-              }
-              gdb::wave_push(push, w, q, tail);
-            },
-            kSerialLimit);
+        if (valid && l > 0) ds[v].y = __float_as_int(sg);
       }
       if (!discovery_barrier(ctx, &local_ctx)) return;
       ++nbar;
@@ -207,24 +241,32 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
       le = t;
       ++l;
     }
-    // q[lb, le) is the deepest level L = l; reached set is q[0, le).
+    // q[lb, le) is the deepest level L = l (delta 0: coef = 1 / sigma); the
+    // reached set is q[0, le).
     const int reached = le;
     levels += l + 1;
-    // Backward: delta[w] for w on level L-1 down to 0, pulled from out-edges
-    // into level +1.  q is ordered by level, so each level's range is found
-    // by binary search on d over q.
+    for (int i = lb + tid; i < le; i += stride) {
+      const int v = q[i];
+      if ((unsigned)v < (unsigned)n) ds[v].y = __float_as_int(1.f / __int_as_float(ds[v].y));
+    }
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    // Backward: delta[w] = sigma[w] * sum of coef over out-edges into level
+    // +1, for w on level L-1 down to 0.  q is ordered by level, so each
+    // level's range is found by binary search on the level words.
     for (int ll = l - 1; ll >= 0; --ll) {
-      // [qb, qe) = positions of level ll in q.
       int lo = 0, hi = reached;
-      while (lo < hi) {  // first position with d >= ll
+      while (lo < hi) {  // first position with level >= ll
         const int mid = (lo + hi) >> 1;
-        if (level_of(d, q, mid, n) < ll) lo = mid + 1; else hi = mid;
+        const int v = q[mid];
+        if (((unsigned)v < (unsigned)n ? ds[v].x : -1) < ll) lo = mid + 1; else hi = mid;
       }
       const int qb = lo;
       hi = reached;
-      while (lo < hi) {  // first position with d >= ll + 1
+      while (lo < hi) {  // first position with level >= ll + 1
         const int mid = (lo + hi) >> 1;
-        if (level_of(d, q, mid, n) < ll + 1) lo = mid + 1; else hi = mid;
+        const int v = q[mid];
+        if (((unsigned)v < (unsigned)n ? ds[v].x : -1) < ll + 1) lo = mid + 1; else hi = mid;
       }
       const int qe = lo;
       GDB_FOR_UNIFORM(i, valid, tid, stride, qe - qb) {
@@ -237,17 +279,20 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
           b = row[w];
           e = row[w + 1];
         }
-        const float sw = valid ? sigma[w] : 0.f;
-        auto edge = [&](int k) -> float {
-          const int v = col[k];
-          return d[v] == ll + 1 ? (1.f + delta[v]) / sigma[v] : 0.f;
-        };
-        auto red = [](float a, float c) { return a + c; };
-        const float acc = gdb::wave_coop_reduce(valid != 0, b, e, 0.f, edge, red, kSerialLimit);
+        const float acc = gdb::wave_coop_reduce(
+            valid != 0, b, e, 0.f,
+            [&](int k) -> float {
+              const int v = col[k];
+              if ((unsigned)v >= (unsigned)n) return 0.f;
+              const int2 x = ld_state(ds, v);
+              return x.x == ll + 1 ? __int_as_float(x.y) : 0.f;
+            },
+            plus, kSerialLimit);
         if (valid) {
+          const float sw = __int_as_float(ds[w].y);
           const float dw = sw * acc;
-          delta[w] = dw;
           if (w != s) bc[w] += dw;
+          ds[w].y = __float_as_int((1.f + dw) / sw);
         }
       }
       if (!discovery_barrier(ctx, &local_ctx)) return;
@@ -261,7 +306,6 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
   }
 }
 
-// ---- relaunch: bfs_kernel (163-203) / backtrack_kernel (221-260) per level.
 __global__ void k_bc_clean(int* d, float* rho, float* delta, int s, int n, int* cont) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {  // clean_1d_array
@@ -326,7 +370,8 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
   if (nsrc) GDB_HIP(hipMemcpy(hsrc.data(), sources, sizeof(int) * nsrc, hipMemcpyDeviceToHost));
   for (int s : hsrc)
     if (s < 0 || s >= n) return GDB_E_INVALID;
-  const size_t bytes = carve_bytes<int>(n) * 2 + carve_bytes<float>(n) * 2 + carve_bytes<int>(32);
+  const size_t bytes = carve_bytes<int>(n) * 2 + carve_bytes<float>(n) * 2 + carve_bytes<int>(32) +
+                       carve_bytes<int2>(n);
   int rc = ensure_workspace(ctx, bytes);
   if (rc) return rc;
   Carve cv(ctx->ws);
@@ -335,6 +380,7 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
   float* rho = cv.take<float>(n);
   float* delta = cv.take<float>(n);
   int* flags = cv.take<int>(32);
+  int2* ds = cv.take<int2>(n);
   rc = fill_i32(ctx, flags, 0, 32);
   if (rc) return rc;
   rc = prepare_launch(ctx);
@@ -385,7 +431,7 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
       if (r2) return r2;
       if (tuned)
         hipLaunchKernelGGL(k_bc_tuned<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col,
-                           row_t, col_t, d, rho, delta, bc, sources, nsrc, q, flags + 4, n, out);
+                           row_t, col_t, ds, bc, sources, nsrc, q, flags + 4, n, out);
       else
         hipLaunchKernelGGL(k_bc_plain<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col,
                            row_t, col_t, d, rho, delta, bc, sources, nsrc, flags, n, out);

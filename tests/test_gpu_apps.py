@@ -183,3 +183,35 @@ def test_verify_distances_kernel(ctx):
     n_bad = oracle.verify_distances_ref(g.row, g.col, g.w, bad)
     assert n_bad > 0
     assert gdb.verify_distances(ctx, d, gdb.DeviceBuffer.from_numpy(bad), weighted=True) == n_bad
+
+
+def test_bc_directed_graph_uses_transpose(ctx):
+    # Arcs u -> v with u < v only: the in-edge CSR differs from the CSR, so
+    # the tuned kernel takes its two-walk path (pull in-edges, push out-edges).
+    g = gdb.gen_rmat(11, 16, seed=12)
+    src = np.repeat(np.arange(g.n, dtype=np.int32), np.diff(g.row))
+    keep = src < g.col
+    row = np.zeros(g.n + 1, np.int32)
+    np.add.at(row, src[keep] + 1, 1)
+    dg_h = gdb.CSR(g.n, np.cumsum(row).astype(np.int32), g.col[keep].astype(np.int32),
+                   g.w[keep].astype(np.int32))
+    gt_h = dg_h.transpose()
+    sources = np.array([0, 1, 2, 5, 9, 17], np.int32)
+    want = oracle.bc_brandes_tb(dg_h.row, dg_h.col, sources)
+    d, dt = dev(dg_h.row, dg_h.col), dev(gt_h.row, gt_h.col)
+    for sched in SCHEDS[:3]:
+        got, _ = gdb.bc(ctx, d, dt, sources, gdb.Opts.make(**sched))
+        bc_close(got, want)
+
+
+def test_bc_symmetric_walk_matches_two_walks(ctx):
+    # The same symmetric graph passed once (one walk per frontier vertex) and
+    # as a separate transpose copy (two walks) gives the same centralities.
+    g = gdb.gen_rmat(12, 16, seed=13)
+    src = gdb.permutation(g.n, 13)[:8]
+    d1, d2 = dev(g.row, g.col), dev(g.row, g.col)
+    o = gdb.Opts.make(variant=gdb.VARIANT_TUNED)
+    a, _ = gdb.bc(ctx, d1, d1, src, o)
+    b, _ = gdb.bc(ctx, d1, d2, src, o)
+    bc_close(a, oracle.bc_brandes_tb(g.row, g.col, src))
+    bc_close(b, oracle.bc_brandes_tb(g.row, g.col, src))
