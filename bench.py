@@ -119,7 +119,7 @@ def bench_c1(args) -> int:
 
     trials = []
     for w in (64, 128, 256, 512, 1024):
-        for gr in (4, 8, 16, 32, 64, 128):
+        for gr in (8, 16, 32, 64, 128, 256, 512):
             if gr <= gdb.max_resident_groups(w, 0):
                 trials.append((min(run(gdb.VARIANT_TUNED, w, gr)[0] for _ in range(3)), w, gr))
     trials.sort()

@@ -15,10 +15,15 @@
 //                       r-1's colouring (v coloured in r-1 iff uncoloured and
 //                       node_value[v] > max_d_{r-1}[v]), so a neighbour's state
 //                       is read as "uncoloured after r-1" from color[] and the
-//                       previous max_d buffer; high-degree vertices are walked
-//                       by a whole wave.
+//                       previous max_d buffer.  Rows of more than 16 edges
+//                       are cut into 64-edge chunks spread over every wave
+//                       of the grid, their partial maxima combined by
+//                       atomicMax on order-preserving int keys (exact: max
+//                       does not depend on order).
 //   relaunch          - color / color2 kernels per round with a host loop
 //                       (color.cpp:294-334).
+#include <algorithm>
+
 #include "graph_common.hpp"
 #include "runtime.hpp"
 
@@ -80,40 +85,67 @@ __global__ __launch_bounds__(WGS) void k_color_plain(gdb_kernel_ctx_t* ctx, cons
   }
 }
 
+// Order-preserving int key of a float (IEEE total order, -0 taken as +0):
+// key(a) > key(b) iff a > b for non-NaN a, b, so max and compares can run on
+// ints and atomicMax combines partial maxima exactly.
+__device__ __forceinline__ int fkey(float f) {
+  const int b = __float_as_int(f + 0.f);
+  return b < 0 ? b ^ 0x7fffffff : b;
+}
+
+constexpr int kHeavyDeg = 16;    // rows longer than this are split into chunks
+constexpr int kChunkEdges = 64;  // edges of a heavy row per chunk (one per lane)
+
 template <int WGS>
 __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
                                                      const int* __restrict__ col,
                                                      const float* __restrict__ nv, int* color,
-                                                     int* stops, float* maxd, int n, int max_rounds,
-                                                     int* out) {
+                                                     int* stops, int* mdk, const int* __restrict__ heavy,
+                                                     int nheavy, const int4* __restrict__ chunks,
+                                                     int nchunks, int n, int max_rounds, int* out) {
   DISCOVERY_PROTOCOL(ctx);
   const int tid = p_get_global_id(ctx, &local_ctx);
   const int stride = p_get_global_size(ctx, &local_ctx);
+  const int lane = gdb::lane_id();
+  const int wave = tid >> 6, nwaves = stride >> 6;
+  const int none = fkey(-1.f);
   int r = 1;
   int nbar = 0;
   for (;;) {
-    const float* md_prev = maxd + (size_t)((r - 1) & 1) * n;
-    float* md_cur = maxd + (size_t)(r & 1) * n;
+    // Three key buffers: round r writes md_cur, reads md_prev, and resets
+    // the heavy rows' entries of md_next (read by nobody this round).
+    const int* md_prev = mdk + (size_t)((r + 2) % 3) * n;
+    int* md_cur = mdk + (size_t)(r % 3) * n;
+    int* md_next = mdk + (size_t)((r + 1) % 3) * n;
     const bool have_prev = r > 1;
     if (tid == 0) gdb::st_rlx(&stops[(r + 1) % 3], 0);
+    for (int h = tid; h < nheavy; h += stride) md_next[heavy[h]] = none;
+    // u counts as uncoloured in round r unless coloured before round r-1 or
+    // by round r-1 itself (node_value[u] > max_d_{r-1}[u]).
+    auto uncoloured = [&](int u) { return color[u] == -1 && !(have_prev && fkey(nv[u]) > md_prev[u]); };
     bool st = false;
     GDB_FOR_UNIFORM(v, valid, tid, stride, n) {
       bool active = false;
       int b = 0, e = 0;
-      if (valid && color[v] == -1) {
-        if (have_prev && nv[v] > md_prev[v]) {
-          color[v] = r - 1;
-        } else {
-          active = true;
-          b = row[v];
-          e = row[v + 1];
+      if (valid) {
+        // All of v's words at once (one memory round trip), then decide.
+        const int cv = color[v];
+        const int kv = fkey(nv[v]), mp = md_prev[v];
+        const int rb = row[v], re = row[v + 1];
+        if (cv == -1) {
+          if (have_prev && kv > mp) {
+            color[v] = r - 1;
+          } else {
+            b = rb;
+            e = re;
+            active = (e - b) <= kHeavyDeg;  // longer rows: the chunk pass below
+          }
         }
       }
       const bool deg1 = (e - b) == 1;
       auto edge = [&](int k) -> gdb::MaxFound {
         const int u = col[k];
-        const bool unc = color[u] == -1 && !(have_prev && nv[u] > md_prev[u]);
-        return unc ? gdb::MaxFound{nv[u], 1} : gdb::MaxFound{-1.f, 0};
+        return uncoloured(u) ? gdb::MaxFound{nv[u], 1} : gdb::MaxFound{-1.f, 0};
       };
       auto red = [](gdb::MaxFound a, gdb::MaxFound c) {
         return gdb::MaxFound{a.mx > c.mx ? a.mx : c.mx, a.found | c.found};
@@ -121,11 +153,37 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
       const gdb::MaxFound res =
           gdb::wave_coop_reduce(active && !deg1, b, e, gdb::MaxFound{-1.f, 0}, edge, red, kSerialLimit);
       if (active) {
-        md_cur[v] = deg1 ? -1.f : res.mx;
+        md_cur[v] = deg1 ? none : fkey(res.mx);
         if (!deg1 && res.found) st = true;
       }
     }
-    if (gdb::wave_any(st) && gdb::lane_id() == 0) gdb::st_rlx(&stops[r % 3], 1);
+    // Long rows, one chunk of kChunkEdges edges per wave step, spread over
+    // every wave of the grid; the chunks of one row meet in md_cur[v] by
+    // atomicMax.
+    for (int c = wave; c < nchunks; c += nwaves) {
+      const int4 ch = chunks[c];  // (row, first edge, end edge)
+      const int v = ch.x;
+      const int k = ch.y + lane;
+      // v's state and the chunk's edge are loaded together.
+      const int u = k < ch.z ? col[k] : -1;
+      if (color[v] != -1 || (have_prev && fkey(nv[v]) > md_prev[v])) continue;  // wave-uniform
+      float mx = -1.f;
+      int found = 0;
+      if (u >= 0 && uncoloured(u)) {
+        found = 1;
+        mx = nv[u];
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        found |= __shfl_xor(found, off, 64);
+      }
+      if (found) {
+        st = true;
+        if (lane == 0) atomicMax(&md_cur[v], fkey(mx));
+      }
+    }
+    if (gdb::wave_any(st) && lane == 0) gdb::st_rlx(&stops[r % 3], 1);
     if (!discovery_barrier(ctx, &local_ctx)) return;
     ++nbar;
     if (gdb::ld_rlx(&stops[r % 3]) == 0) break;
@@ -136,9 +194,9 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
     ++r;
   }
   // Round r's own colouring (the reference's color2 of the last round).
-  const float* md = maxd + (size_t)(r & 1) * n;
+  const int* md = mdk + (size_t)(r % 3) * n;
   for (int v = tid; v < n; v += stride)
-    if (color[v] == -1 && nv[v] > md[v]) color[v] = r;
+    if (color[v] == -1 && fkey(nv[v]) > md[v]) color[v] = r;
   if (tid == 0) {
     out[0] = r;
     out[1] = nbar;
@@ -184,12 +242,18 @@ __global__ void k_color_r2(const float* __restrict__ nv, int* color, int* flags,
   if (i == 0) flags[(r + 1) % 3] = 0;
 }
 
-__global__ void k_color_init(int* color, float* max_d, int* flags, int n) {
+__global__ void k_color_init(int* color, float* max_d, int* mdk, int* flags, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     color[i] = -1;
     max_d[i] = -1.f;
     max_d[i + n] = -1.f;
+    if (mdk) {
+      const int none = fkey(-1.f);
+      mdk[i] = none;
+      mdk[i + n] = none;
+      mdk[i + 2 * (size_t)n] = none;
+    }
   }
   if (i < 8) flags[i] = 0;
 }
@@ -211,15 +275,47 @@ extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* 
     if (stats) *stats = s;
     return GDB_SUCCESS;
   }
-  const size_t bytes = carve_bytes<float>((size_t)2 * n) + carve_bytes<int>(16);
+  const bool tuned_persistent = o.variant == GDB_VARIANT_TUNED && o.schedule != GDB_SCHED_RELAUNCH;
+  // Tuned schedule: rows longer than kHeavyDeg are split into chunks of
+  // kChunkEdges edges; the rows and the (row, first edge) of every chunk
+  // are listed here from the row offsets.
+  std::vector<int> heavy;
+  std::vector<int4> chunks;
+  if (tuned_persistent) {
+    std::vector<int> hrow((size_t)n + 1);
+    GDB_HIP(hipMemcpy(hrow.data(), row, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    for (int v = 0; v < n; ++v) {
+      const int d = hrow[v + 1] - hrow[v];
+      if (d < 0 || hrow[v] < 0 || hrow[v + 1] > m) return GDB_E_INVALID;
+      if (d > kHeavyDeg) {
+        heavy.push_back(v);
+        for (int eb = hrow[v]; eb < hrow[v + 1]; eb += kChunkEdges)
+          chunks.push_back(make_int4(v, eb, std::min(eb + kChunkEdges, hrow[v + 1]), 0));
+      }
+    }
+  }
+  const int nheavy = (int)heavy.size();
+  const int nchunks = (int)chunks.size();
+  const size_t bytes = carve_bytes<float>((size_t)2 * n) + carve_bytes<int>(16) +
+                       (tuned_persistent ? carve_bytes<int>((size_t)3 * n) + carve_bytes<int>(nheavy + 1) +
+                                               carve_bytes<int4>(nchunks + 1)
+                                         : 0);
   int rc = ensure_workspace(ctx, bytes);
   if (rc) return rc;
   Carve cv(ctx->ws);
   float* maxd = cv.take<float>((size_t)2 * n);
   int* flags = cv.take<int>(16);
+  int* mdk = tuned_persistent ? cv.take<int>((size_t)3 * n) : nullptr;
+  int* d_heavy = tuned_persistent ? cv.take<int>(nheavy + 1) : nullptr;
+  int4* d_chunks = tuned_persistent ? cv.take<int4>(nchunks + 1) : nullptr;
+  if (nheavy) {
+    GDB_HIP(hipMemcpyAsync(d_heavy, heavy.data(), sizeof(int) * nheavy, hipMemcpyHostToDevice, ctx->stream));
+    GDB_HIP(hipMemcpyAsync(d_chunks, chunks.data(), sizeof(int4) * nchunks, hipMemcpyHostToDevice,
+                           ctx->stream));
+  }
   const int max_rounds = n + 1;
   const int init_groups = (n + 255) / 256;
-  hipLaunchKernelGGL(k_color_init, dim3(init_groups), dim3(256), 0, ctx->stream, color, maxd, flags, n);
+  hipLaunchKernelGGL(k_color_init, dim3(init_groups), dim3(256), 0, ctx->stream, color, maxd, mdk, flags, n);
   GDB_HIP(hipGetLastError());
   rc = prepare_launch(ctx);
   if (rc) return rc;
@@ -271,7 +367,8 @@ extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* 
       if (r2) return r2;
       if (tuned)
         hipLaunchKernelGGL(k_color_tuned<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row,
-                           col, node_value, color, flags, maxd, n, max_rounds, out);
+                           col, node_value, color, flags, mdk, d_heavy, nheavy, d_chunks, nchunks, n,
+                           max_rounds, out);
       else
         hipLaunchKernelGGL(k_color_plain<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row,
                            col, node_value, color, flags, flags + 4, maxd, n, max_rounds, out);

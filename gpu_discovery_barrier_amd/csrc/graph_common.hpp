@@ -32,8 +32,11 @@ __device__ __forceinline__ T wave_coop_reduce(bool active, int b, int e, T ident
                                               RedFn red, int serial_limit) {
   T acc = identity;
   const bool heavy = active && (e - b) > serial_limit;
-  if (active && !heavy)
+  // Unrolled so that the loads of several edges are in flight together.
+  if (active && !heavy) {
+#pragma unroll 4
     for (int k = b; k < e; ++k) acc = red(acc, edge(k));
+  }
   unsigned long long mask = __ballot(heavy);
   const int lane = lane_id();
   while (mask) {
@@ -41,6 +44,7 @@ __device__ __forceinline__ T wave_coop_reduce(bool active, int b, int e, T ident
     const int lb = __shfl(b, leader, 64);
     const int le = __shfl(e, leader, 64);
     T part = identity;
+#pragma unroll 4
     for (int k = lb + lane; k < le; k += 64) part = red(part, edge(k));
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) part = red(part, shfl_xor(part, off));
