@@ -33,6 +33,7 @@
 namespace {
 
 constexpr int kSerialLimit = 16;
+constexpr int kChunk = 4;  // edges a lane handles per step in the tuned BFS walk
 
 __device__ __forceinline__ void fatomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 
@@ -177,27 +178,52 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
           b = row[v];
           e = row[v + 1];
         }
-        // Claims w for level l+1 if it is unreached (all lanes call it).
-        auto claim = [&](bool ok, int w, int lvl) {
-          const bool push = ok && lvl == -1 && atomicCAS(&ds[w].x, -1, l + 1) == -1;
-          gdb::wave_push(push, w, q, tail);
-        };
-        float sg = 0.f;
-        if (sym) {
-          sg = gdb::wave_coop_walk(
-              valid != 0, b, e, 0.f,
+        // Walks v's out-edges (light rows four edges per lane step, so that
+        // four gathers and claims are in flight and one wave push serves
+        // them; longer rows by the whole wave): claims unreached targets for
+        // level l+1 and, when `pull_sigma` (a symmetric graph), sums sigma
+        // over the targets on level l-1.
+        auto walk = [&](bool pull_sigma) -> float {
+          const bool heavy = valid && (e - b) > kSerialLimit;
+          const int len = (valid && !heavy) ? (e - b) : 0;
+          const int maxlen = gdb::wave_max(len);
+          float acc = 0.f;
+          for (int c = 0; c < maxlen; c += kChunk) {
+            int w[kChunk];
+            bool push[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) w[j] = c + j < len ? col[b + c + j] : -1;
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+              push[j] = false;
+              if ((unsigned)w[j] < (unsigned)n) {
+                const int2 x = ld_state(ds, w[j]);
+                if (pull_sigma && x.x == l - 1) acc += __int_as_float(x.y);
+                push[j] = x.x == -1 && atomicCAS(&ds[w[j]].x, -1, l + 1) == -1;
+              }
+            }
+            gdb::wave_push_n<kChunk>(push, w, q, tail);
+          }
+          const float hv = gdb::wave_coop_walk(
+              heavy, b, e, 0.f,
               [&](int k, bool ok) -> float {
-                int w = 0;
+                int w = -1;
                 int2 x = make_int2(0, 0);
                 if (ok) {
                   w = col[k];
                   ok = (unsigned)w < (unsigned)n;
                   if (ok) x = ld_state(ds, w);
                 }
-                claim(ok, w, x.x);
-                return (ok && l > 0 && x.x == l - 1) ? __int_as_float(x.y) : 0.f;
+                const bool push = ok && x.x == -1 && atomicCAS(&ds[w].x, -1, l + 1) == -1;
+                gdb::wave_push(push, w, q, tail);
+                return (ok && pull_sigma && x.x == l - 1) ? __int_as_float(x.y) : 0.f;
               },
-              plus, kSerialLimit);
+              plus, -1);
+          return heavy ? hv : acc;
+        };
+        float sg = 0.f;
+        if (sym) {
+          sg = walk(l > 0);
         } else {
           int bt = 0, et = 0;
           if (valid && l > 0) {
@@ -213,19 +239,7 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
                 return x.x == l - 1 ? __int_as_float(x.y) : 0.f;
               },
               plus, kSerialLimit);
-          gdb::wave_coop_walk(
-              valid != 0, b, e, 0.f,
-              [&](int k, bool ok) -> float {
-                int w = 0, lvl = 0;
-                if (ok) {
-                  w = col[k];
-                  ok = (unsigned)w < (unsigned)n;
-                  if (ok) lvl = ds[w].x;
-                }
-                claim(ok, w, lvl);
-                return 0.f;
-              },
-              plus, kSerialLimit);
+          walk(false);
         }
         if (valid && l > 0) ds[v].y = __float_as_int(sg);
       }
