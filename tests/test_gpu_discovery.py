@@ -133,3 +133,24 @@ def test_time_protocol(ctx):
     ns = ctx.time_protocol(1000, 256)
     print("protocol ns", ns, "P", ctx.participating())
     assert ns > 0
+
+
+def test_occupancy_suite():
+    # The counterpart of run_occupancy_tests.py / run_occupancy_timing.py,
+    # two runs per configuration and lock kind.
+    from gpu_discovery_barrier_amd import occupancy_suite
+
+    res = occupancy_suite.run(iterations=2, groups=1000)
+    text = occupancy_suite.report(res)
+    print(text)
+    assert len(res["configs"]) == 4
+    for d in res["configs"]:
+        assert d["occupancy"] > 0 and d["skip_mode_at_bound_completes"] and d["one_more_refused"]
+        for name, _ in occupancy_suite.LOCKS:
+            r = d[name]
+            assert len(r["occupancy_list"]) == 2 and r["std_dev"] >= 0
+            assert all(1 <= p <= min(1000, d["occupancy"]) for p in r["occupancy_list"])
+            assert r["protocol_ms_average"] > 0
+        # The lock-free protocol admits every co-resident group it can see.
+        assert d["lock-free"]["average"] >= min(1000, d["occupancy"]) * 0.5
+    assert "true occupancy:" in text and "ticket occupancy list:" in text
