@@ -177,6 +177,8 @@ def _declare(lib) -> None:
         "gdb_bfs_worklist": (I, [P, I, I, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_sssp_worklist": (I, [P, I, I, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_verify_distances": (I, [P, I, I, P, P, P, P, C.POINTER(C.c_longlong)]),
+        "gdb_mst": (I, [P, I, I, P, P, P, C.POINTER(C.c_ulonglong), C.POINTER(I), C.POINTER(I),
+                        C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_batch_solve": (I, [P, C.POINTER(Task), I, I, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_batch_solve_sel": (I, [P, C.POINTER(Task), I, I, C.POINTER(Opts), C.POINTER(Selector),
                                     C.POINTER(Stats)]),
@@ -519,6 +521,18 @@ def sssp_worklist(ctx: Context, g: DeviceCSR, source: int = 0, opts=None,
     check(lib().gdb_sssp_worklist(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr, g.w.ptr, source, out.ptr,
                                   C.byref(o), C.byref(st)), "gdb_sssp_worklist")
     return out.to_numpy(), st
+
+
+def mst(ctx: Context, g: DeviceCSR, opts=None):
+    """Minimum spanning forest of a symmetric graph: (weight, edges, components, Stats)."""
+    if g.w is None:
+        raise ValueError("mst needs edge weights")
+    wt, ed, cp = C.c_ulonglong(0), C.c_int(0), C.c_int(0)
+    st = Stats()
+    o = _opts(opts)
+    check(lib().gdb_mst(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr, g.w.ptr, C.byref(wt), C.byref(ed), C.byref(cp),
+                        C.byref(o), C.byref(st)), "gdb_mst")
+    return wt.value, ed.value, cp.value, st
 
 
 def verify_distances(ctx: Context, g: DeviceCSR, dist: DeviceBuffer, weighted: bool) -> int:

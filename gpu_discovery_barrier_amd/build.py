@@ -50,6 +50,7 @@ TOOLS = [
     ("bc-gb", "cli/pannotia_app.cpp", ["-DGDB_APP_BC"]),
     ("bfs-port", "cli/lonestar_app.cpp", ["-DGDB_APP_BFS"]),
     ("sssp-port", "cli/lonestar_app.cpp", ["-DGDB_APP_SSSP"]),
+    ("mst-port", "cli/lonestar_app.cpp", ["-DGDB_APP_MST"]),
     ("occupancy_test", "cli/occupancy_tools.cpp", ["-DGDB_TOOL_OCCUPANCY"]),
     ("time_prot", "cli/occupancy_tools.cpp", ["-DGDB_TOOL_TIME_PROT"]),
     ("device_query", "cli/occupancy_tools.cpp", ["-DGDB_TOOL_DEVICE_QUERY"]),

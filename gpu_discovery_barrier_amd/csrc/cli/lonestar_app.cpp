@@ -1,13 +1,15 @@
-// Lonestar command-line tools: bfs-port, sssp-port.  Usage and stdout
-// strings of lonestar_gpu_opencl/apps/{bfs,sssp}/*_port.cpp:
+// Lonestar command-line tools: bfs-port, sssp-port, mst-port.  Usage and
+// stdout strings of lonestar_gpu_opencl/apps/{bfs,sssp,mst}/*_port.cpp:
 // `<exe> <graph.gr|graph.edges> <wgs>` (wgs a power of two); source 0.
 // Environment: GDB_VARIANT=plain|tuned, GDB_GROUPS=<n>.
 #include <cstring>
 #include <vector>
 #include "cli_common.h"
 
-#ifdef GDB_APP_BFS
+#if defined(GDB_APP_BFS)
 #define EXE "bfs-port"
+#elif defined(GDB_APP_MST)
+#define EXE "mst-port"
 #else
 #define EXE "sssp-port"
 #endif
@@ -32,9 +34,25 @@ int main(int argc, char** argv) {
   check(gdb_ctx_create(&ctx), "gdb_ctx_create");
   check(gdb_ctx_init(ctx), "init_discovery_kernel_ctx");
   DevCSR d = upload(g);
-  unsigned* dist = (unsigned*)dalloc(sizeof(unsigned) * (g.n ? g.n : 1));
   gdb_opts o = opts_from_env(wgs, GDB_SCHED_PERSISTENT);
   gdb_stats st{};
+#ifdef GDB_APP_MST
+  // mst_port.cpp:202-313: rounds until the component count stops falling;
+  // prints weight (u32, as the reference's cl_uint), components and edges.
+  printf("finding mst.\n");
+  unsigned long long weight = 0;
+  int edges = 0, comps = 0;
+  check(gdb_mst(ctx, g.n, g.m, d.row, d.col, (const unsigned*)d.w, &weight, &edges, &comps, &o, &st), "gdb_mst");
+  printf("\t%s result: weight: %u, components: %u, edges: %u\n", file, (unsigned)weight, (unsigned)comps,
+         (unsigned)edges);
+  printf("\tapp runtime = %f ms.\n", st.kernel_ms);
+  printf("\tnumber of participating groups = %d\n", st.participating);
+  release(d);
+  gdb_csr_free(&g);
+  gdb_ctx_destroy(ctx);
+  return 0;
+#endif
+  unsigned* dist = (unsigned*)dalloc(sizeof(unsigned) * (g.n ? g.n : 1));
   printf("solving.\n");
   printf("starting...\n");
 #ifdef GDB_APP_BFS

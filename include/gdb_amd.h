@@ -173,6 +173,17 @@ int gdb_sssp_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col
 int gdb_verify_distances(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* w,
                          const unsigned* dist, long long* nerr);
 
+/* Minimum spanning forest (Boruvka) in one persistent launch.  Replaces the
+ * lonestar mst-port host loop and kernels (lonestar_gpu_opencl/apps/mst/
+ * mst_port.cpp:203-313; kernel/mst_common.cl, mst_port_kernel.cl).  The graph
+ * must be symmetric (each undirected edge stored in both directions, as the
+ * reference's .sym.gr inputs).  Outputs: total weight of the forest, its
+ * edge count and the number of components (isolated vertices included), as
+ * mst-port prints them ("weight: %u, components: %u, edges: %u"). */
+int gdb_mst(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const unsigned* w,
+            unsigned long long* weight, int* edges, int* components, const gdb_opts* opts,
+            gdb_stats* stats);
+
 /* ---- batched engine -------------------------------------------------------
  * One persistent launch solves many (graph, algorithm) tasks: discovered
  * groups form teams of `team_groups` groups; each team takes tasks from a

@@ -51,6 +51,8 @@ def lib():
             "orc_block_scan_ref": (I, [I, P, P]),
             "orc_dp_model": (I, [I, I, I, I, I, I, I, C.POINTER(I), P, C.POINTER(C.c_longlong)]),
             "orc_mlp_forward": (None, [P, P, P]),
+            "orc_mst_kruskal": (I, [I, P, P, P, C.POINTER(C.c_ulonglong), C.POINTER(I), C.POINTER(I)]),
+            "orc_mst_prim_tb": (I, [I, P, P, P, C.POINTER(C.c_ulonglong), C.POINTER(I), C.POINTER(I)]),
             "orc_mlp_grad": (I, [P, P, I, P, C.POINTER(C.c_double)]),
         }
         for name, (res, args) in sig.items():
@@ -222,3 +224,20 @@ def mlp_grad(w, records):
     loss = C.c_double(0)
     used = lib().orc_mlp_grad(_p(w), _p(rec), rec.shape[0], _p(g), C.byref(loss))
     return g, loss.value, used
+
+
+def _mst(fn, row, col, w):
+    row, col = _i32(row), _i32(col)
+    w = np.ascontiguousarray(w, dtype=np.uint32)
+    wt, ed, cp = C.c_ulonglong(0), C.c_int(0), C.c_int(0)
+    fn(row.size - 1, _p(row), _p(col), _p(w), C.byref(wt), C.byref(ed), C.byref(cp))
+    return wt.value, ed.value, cp.value
+
+
+def mst_kruskal(row, col, w):
+    """(forest weight, edges, components) of a symmetric CSR."""
+    return _mst(lib().orc_mst_kruskal, row, col, w)
+
+
+def mst_prim_tb(row, col, w):
+    return _mst(lib().orc_mst_prim_tb, row, col, w)

@@ -956,3 +956,122 @@ int orc_mlp_grad(const double* w, const float* records, int count, double* grad,
   *loss = used > 0 ? lsum / used : 0;
   return used;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Minimum spanning forest                                                   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  unsigned w;
+  int a, b;
+} orc_edge;
+
+static int orc_edge_cmp(const void* x, const void* y) {
+  const orc_edge* p = (const orc_edge*)x;
+  const orc_edge* q = (const orc_edge*)y;
+  if (p->w != q->w) return p->w < q->w ? -1 : 1;
+  if (p->a != q->a) return p->a < q->a ? -1 : 1;
+  return (p->b > q->b) - (p->b < q->b);
+}
+
+static int uf_find(int* par, int x) {
+  while (par[x] != x) {
+    par[x] = par[par[x]];
+    x = par[x];
+  }
+  return x;
+}
+
+/* lonestar apps/mst/kernel/mst_common.cl (Boruvka); any algorithm gives the
+ * same forest weight.  Each undirected edge is taken once (a < b). */
+int orc_mst_kruskal(int n, const int* row, const int* col, const unsigned* w, unsigned long long* weight,
+                    int* edges, int* components) {
+  long long me = 0;
+  for (int v = 0; v < n; ++v)
+    for (int k = row[v]; k < row[v + 1]; ++k)
+      if (v < col[k]) ++me;
+  orc_edge* e = (orc_edge*)malloc(sizeof(orc_edge) * (size_t)(me > 0 ? me : 1));
+  int* par = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  long long c = 0;
+  for (int v = 0; v < n; ++v)
+    for (int k = row[v]; k < row[v + 1]; ++k)
+      if (v < col[k]) {
+        e[c].w = w[k];
+        e[c].a = v;
+        e[c].b = col[k];
+        ++c;
+      }
+  qsort(e, (size_t)me, sizeof(orc_edge), orc_edge_cmp);
+  for (int v = 0; v < n; ++v) par[v] = v;
+  unsigned long long total = 0;
+  int taken = 0;
+  for (long long i = 0; i < me; ++i) {
+    const int ra = uf_find(par, e[i].a), rb = uf_find(par, e[i].b);
+    if (ra == rb) continue;
+    par[ra] = rb;
+    total += e[i].w;
+    ++taken;
+  }
+  *weight = total;
+  *edges = taken;
+  *components = n - taken;
+  free(e);
+  free(par);
+  return 0;
+}
+
+/* Prim with a binary heap of (weight, vertex), restarted from every vertex
+ * not yet in a tree. */
+int orc_mst_prim_tb(int n, const int* row, const int* col, const unsigned* w, unsigned long long* weight,
+                    int* edges, int* components) {
+  const long long m = n > 0 ? row[n] : 0;
+  unsigned long long* heap = (unsigned long long*)malloc(sizeof(unsigned long long) * (size_t)(m + n + 1));
+  char* in = (char*)calloc((size_t)(n > 0 ? n : 1), 1);
+  unsigned long long total = 0;
+  int taken = 0, comps = 0;
+  for (int s = 0; s < n; ++s) {
+    if (in[s]) continue;
+    ++comps;
+    long long hs = 0;
+    heap[hs++] = (unsigned long long)s;  /* weight 0, vertex s */
+    while (hs > 0) {
+      /* pop min */
+      const unsigned long long top = heap[0];
+      heap[0] = heap[--hs];
+      for (long long i = 0;;) {
+        long long l = 2 * i + 1, r = l + 1, sm = i;
+        if (l < hs && heap[l] < heap[sm]) sm = l;
+        if (r < hs && heap[r] < heap[sm]) sm = r;
+        if (sm == i) break;
+        const unsigned long long t = heap[i];
+        heap[i] = heap[sm];
+        heap[sm] = t;
+        i = sm;
+      }
+      const int v = (int)(top & 0xffffffffu);
+      if (in[v]) continue;
+      in[v] = 1;
+      if (v != s) {
+        total += top >> 32;
+        ++taken;
+      }
+      for (int k = row[v]; k < row[v + 1]; ++k) {
+        const int u = col[k];
+        if (u < 0 || u >= n || in[u]) continue;
+        long long i = hs++;
+        heap[i] = ((unsigned long long)w[k] << 32) | (unsigned)u;
+        while (i > 0 && heap[(i - 1) / 2] > heap[i]) {
+          const unsigned long long t = heap[i];
+          heap[i] = heap[(i - 1) / 2];
+          heap[(i - 1) / 2] = t;
+          i = (i - 1) / 2;
+        }
+      }
+    }
+  }
+  *weight = total;
+  *edges = taken;
+  *components = comps;
+  free(heap);
+  free(in);
+  return 0;
+}

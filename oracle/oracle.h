@@ -97,6 +97,17 @@ void orc_mlp_forward(const double* w, const double* x, double* y);
  * (5508 doubles) by backpropagation.  Returns the number of records used. */
 int orc_mlp_grad(const double* w, const float* records, int count, double* grad, double* loss);
 
+/* ---- minimum spanning forest ---------------------------------------------
+ * Kruskal (edges sorted by weight, union-find) over the undirected edges of
+ * a symmetric CSR; the forest weight is unique.  Restates what lonestar mst
+ * computes (apps/mst/kernel/mst_common.cl; mst_port.cpp:300-313 prints
+ * weight, components, edges).  Returns 0. */
+int orc_mst_kruskal(int n, const int* row, const int* col, const unsigned* w, unsigned long long* weight,
+                    int* edges, int* components);
+/* Prim from every unvisited vertex (binary heap): the second textbook form. */
+int orc_mst_prim_tb(int n, const int* row, const int* col, const unsigned* w, unsigned long long* weight,
+                    int* edges, int* components);
+
 #ifdef __cplusplus
 }
 #endif
