@@ -23,6 +23,8 @@
 // group that completes each team barrier picks the next iteration's arm
 // (selector.hpp) and publishes it with the release.  All arms reach the same
 // fixpoint, so outputs do not depend on the choices.
+#include <algorithm>
+
 #include "graph_common.hpp"
 #include "runtime.hpp"
 #include "selector.hpp"
@@ -868,11 +870,23 @@ extern "C" int gdb_batch_solve_sel(gdb_ctx* ctx, gdb_task* tasks, int ntasks, in
     if (stats) *stats = st;
     return GDB_SUCCESS;
   }
+  // The device task table lists the tasks longest first (teams take them in
+  // table order, so the large ones do not start last and stretch the tail):
+  // estimated cost (n + m) x {SSSP 3, BFS 2, colour 1}.  order[i] is the
+  // caller's index of table entry i.
+  std::vector<int> order((size_t)ntasks);
+  for (int t = 0; t < ntasks; ++t) order[t] = t;
+  auto cost = [&](int t) {
+    const gdb_task& a = tasks[t];
+    const long long f = a.algo == GDB_TASK_SSSP ? 3 : a.algo == GDB_TASK_BFS ? 2 : 1;
+    return ((long long)a.n + a.m) * f;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
   // Per-task scratch sizes (ints) and the device task table.
   std::vector<DevTask> dt((size_t)ntasks);
   size_t scratch_total = 0;
   for (int t = 0; t < ntasks; ++t) {
-    const gdb_task& a = tasks[t];
+    const gdb_task& a = tasks[order[t]];
     if (a.n < 0 || a.m < 0 || a.algo < GDB_TASK_BFS || a.algo > GDB_TASK_COLOR || !a.out ||
         (a.n > 0 && !a.row) || (a.m > 0 && !a.col) || (a.algo == GDB_TASK_SSSP && a.m > 0 && !a.w) ||
         (a.algo == GDB_TASK_COLOR && a.n > 0 && !a.node_value) ||
@@ -940,18 +954,19 @@ extern "C" int gdb_batch_solve_sel(gdb_ctx* ctx, gdb_task* tasks, int ntasks, in
   std::vector<unsigned long long> hwork((size_t)8 * ntasks);
   GDB_HIP(hipMemcpy(hres.data(), results, sizeof(int) * ntasks, hipMemcpyDeviceToHost));
   GDB_HIP(hipMemcpy(hwork.data(), work, sizeof(unsigned long long) * 8 * ntasks, hipMemcpyDeviceToHost));
-  for (int t = 0; t < ntasks; ++t) {
-    const unsigned long long* wk = &hwork[(size_t)8 * t];
-    tasks[t].rounds = hres[t];
-    st.rounds += hres[t];
+  for (int i = 0; i < ntasks; ++i) {
+    const unsigned long long* wk = &hwork[(size_t)8 * i];
+    gdb_task& tk = tasks[order[i]];
+    tk.rounds = hres[i];
+    st.rounds += hres[i];
     // Worklist tasks report items / edges processed; colour tasks count the
     // graph once.
-    const bool wl = tasks[t].algo != GDB_TASK_COLOR;
-    st.work_items += wl ? (long long)wk[0] : tasks[t].n;
-    st.work_edges += wl ? (long long)wk[1] : tasks[t].m;
+    const bool wl = tk.algo != GDB_TASK_COLOR;
+    st.work_items += wl ? (long long)wk[0] : tk.n;
+    st.work_edges += wl ? (long long)wk[1] : tk.m;
     // Device wall clock (100 MHz) from the team's start to end of the task.
-    tasks[t].time_us = (double)(wk[3] - wk[2]) / 100.0;
-    for (int a = 0; a < 4; ++a) tasks[t].arm_iters[a] = (int)wk[4 + a];
+    tk.time_us = (double)(wk[3] - wk[2]) / 100.0;
+    for (int a = 0; a < 4; ++a) tk.arm_iters[a] = (int)wk[4 + a];
   }
   st.launches = 1;
   rc = read_participating(ctx, &st.participating);
