@@ -271,6 +271,11 @@ def bench_c4(args) -> int:
     _, wgs, groups, delta = trials[0]
     log(f"autotune: wgs={wgs} groups={groups} delta={delta} ({trials[0][0]:.1f} ms)")
     k_ms, wall, (_, st) = timed(lambda: run(wgs, groups, 1, delta), args.warmup, args.steps)
+    # The same batch captured once as a HIP graph and replayed.
+    gtasks = tasks(delta)
+    graph = gdb.BatchGraph(ctx, gtasks, 1, gdb.Opts.make(wgs=wgs, groups=groups))
+    g_ms, g_wall, _ = timed(lambda: (graph.launch()[1].kernel_ms, None), args.warmup, args.steps)
+    graph.close()
     # Reached edges (BFS and SSSP reach the same vertices) and parity on a sample.
     deg_r = []
     exact = True
@@ -298,6 +303,7 @@ def bench_c4(args) -> int:
              {"workload": f"4096 graphs (2048 G(n,p) + 2048 grids, n={N}, m={M}) x BFS, SSSP, colour",
               "wgs": wgs, "groups": groups, "team_groups": 1, "delta": delta, "participating": st.participating},
              "u32", {"kernel_ms": round(k_ms, 3), "work_edges": work,
+                     "graph_replay": {"ms": round(g_ms, 3), "wall_ms": round(g_wall, 3), "eager_wall_ms": round(wall, 3)},
                      "plain": {"gteps": round(plain_gteps, 4), "sample": "every 16th graph, one task per launch",
                                "ms_sample": round(plain_ms, 3)},
                      "speedup_vs_plain": round(gteps / plain_gteps, 3), "bit_exact_sample": exact})

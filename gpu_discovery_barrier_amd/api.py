@@ -182,6 +182,10 @@ def _declare(lib) -> None:
         "gdb_batch_solve": (I, [P, C.POINTER(Task), I, I, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_batch_solve_sel": (I, [P, C.POINTER(Task), I, I, C.POINTER(Opts), C.POINTER(Selector),
                                     C.POINTER(Stats)]),
+        "gdb_batch_graph_create": (I, [P, C.POINTER(Task), I, I, C.POINTER(Opts), C.POINTER(Selector),
+                                       C.POINTER(P)]),
+        "gdb_batch_graph_launch": (I, [P, C.POINTER(Stats)]),
+        "gdb_batch_graph_destroy": (I, [P]),
         "gdb_mlp_forward": (I, [P, P, P, I, P]),
         "gdb_mlp_grad": (I, [P, P, P, I, P, C.POINTER(C.c_float), C.POINTER(I)]),
         "gdb_mlp_to_bf16": (I, [P, P, P, I]),
@@ -601,6 +605,41 @@ def batch_solve(ctx: Context, tasks: list, team_groups: int = 1, opts=None, sele
         t.rounds, t.time_us = arr[i].rounds, arr[i].time_us
         t.arm_iters[:] = arr[i].arm_iters[:]
     return [arr[i].rounds for i in range(len(tasks))], st
+
+
+class BatchGraph:
+    """A batch recorded once as a HIP graph and replayed (gdb_batch_graph_*).
+    Keeps its own copy of the task array; replays update those tasks'
+    rounds / time_us / arm_iters and the caller's Task objects."""
+
+    def __init__(self, ctx: Context, tasks: list, team_groups: int = 1, opts=None,
+                 selector: "Selector | None" = None):
+        self.tasks = tasks
+        self.arr = (Task * len(tasks))(*tasks)
+        self.h = P()
+        o = _opts(opts)
+        check(lib().gdb_batch_graph_create(ctx.h, self.arr, len(tasks), team_groups, C.byref(o),
+                                           C.byref(selector) if selector is not None else None, C.byref(self.h)),
+              "gdb_batch_graph_create")
+
+    def launch(self):
+        st = Stats()
+        check(lib().gdb_batch_graph_launch(self.h, C.byref(st)), "gdb_batch_graph_launch")
+        for i, t in enumerate(self.tasks):
+            t.rounds, t.time_us = self.arr[i].rounds, self.arr[i].time_us
+            t.arm_iters[:] = self.arr[i].arm_iters[:]
+        return [self.arr[i].rounds for i in range(len(self.tasks))], st
+
+    def close(self) -> None:
+        if self.h:
+            lib().gdb_batch_graph_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

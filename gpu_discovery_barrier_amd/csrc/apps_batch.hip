@@ -851,30 +851,47 @@ __global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevT
 
 using namespace gdbrt;
 
-extern "C" int gdb_batch_solve_sel(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups,
-                                   const gdb_opts* opts, const gdb_selector* sel, gdb_stats* stats) {
-  if (!ctx || ntasks < 0 || (ntasks > 0 && !tasks) || team_groups < 0) return GDB_E_INVALID;
+namespace {
+
+// A batch laid out on the device: the task table (longest first), the team
+// state, counters, per-task work words and scratch, in `mem` (the context's
+// workspace, or an allocation of its own for a captured graph).
+struct BatchPlan {
+  gdb_ctx* ctx = nullptr;
+  int ntasks = 0, G = 1, groups = 0, wgs = 256;
+  size_t dyn_lds = 0;
+  gdbsel::Cfg cfg{GDB_SEL_NONE, 0, 1, 0u, nullptr, nullptr, 0, nullptr};
+  std::vector<int> order;  // order[i] = caller's index of table entry i
+  DevTask* d_tasks = nullptr;
+  int* team_state = nullptr;
+  size_t team_ints = 0;
+  int* misc = nullptr;
+  unsigned long long* work = nullptr;
+  void* own = nullptr;  // allocation owned by a graph handle
+};
+
+int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups, const gdb_opts* opts,
+               const gdb_selector* sel, bool own_memory, BatchPlan* P) {
+  if (!ctx || ntasks <= 0 || !tasks || team_groups < 0) return GDB_E_INVALID;
   gdb_opts o{};
   if (opts) o = *opts;
-  const int wgs = default_wgs(&o);
-  gdbsel::Cfg cfg{GDB_SEL_NONE, 0, 1, 0u, nullptr, nullptr, 0, nullptr};
+  P->ctx = ctx;
+  P->ntasks = ntasks;
+  P->wgs = default_wgs(&o);
+  P->G = team_groups > 0 ? team_groups : 1;
   if (sel) {
     if (sel->mode < GDB_SEL_NONE || sel->mode > GDB_SEL_MLP || sel->narrow_groups < 0 ||
         sel->trace_capacity < 0 || (sel->trace && !sel->trace_count))
       return GDB_E_INVALID;
-    cfg = gdbsel::Cfg{sel->mode, sel->fixed_arm, sel->narrow_groups, sel->seed, sel->weights,
-                      sel->trace, sel->trace ? sel->trace_capacity : 0, sel->trace_count};
+    P->cfg = gdbsel::Cfg{sel->mode, sel->fixed_arm, sel->narrow_groups, sel->seed, sel->weights,
+                         sel->trace, sel->trace ? sel->trace_capacity : 0, sel->trace_count};
   }
-  gdb_stats st{};
-  if (ntasks == 0) {
-    if (stats) *stats = st;
-    return GDB_SUCCESS;
-  }
+  const gdbsel::Cfg& cfg = P->cfg;
   // The device task table lists the tasks longest first (teams take them in
   // table order, so the large ones do not start last and stretch the tail):
-  // estimated cost (n + m) x {SSSP 3, BFS 2, colour 1}.  order[i] is the
-  // caller's index of table entry i.
-  std::vector<int> order((size_t)ntasks);
+  // estimated cost (n + m) x {SSSP 3, BFS 2, colour 1}.
+  std::vector<int>& order = P->order;
+  order.resize((size_t)ntasks);
   for (int t = 0; t < ntasks; ++t) order[t] = t;
   auto cost = [&](int t) {
     const gdb_task& a = tasks[t];
@@ -899,25 +916,31 @@ extern "C" int gdb_batch_solve_sel(gdb_ctx* ctx, gdb_task* tasks, int ntasks, in
                     nullptr, a.trow, a.tcol, a.tw};
     scratch_total += (ints + 63) & ~(size_t)63;
   }
-  const size_t dyn_lds = (cfg.mode == GDB_SEL_MLP && cfg.w) ? sizeof(unsigned short) * gdbsel::kParams : 0;
-  int groups = 0;
-  int rc = gdb_dispatch_wgs(wgs, [&](auto W) -> int {
+  P->dyn_lds = (cfg.mode == GDB_SEL_MLP && cfg.w) ? sizeof(unsigned short) * gdbsel::kParams : 0;
+  int rc = gdb_dispatch_wgs(P->wgs, [&](auto W) -> int {
     constexpr int WG = decltype(W)::value;
     const void* kern = cfg.mode != GDB_SEL_NONE ? (const void*)k_batch<WG, true> : (const void*)k_batch<WG, false>;
-    return launch_groups(ctx, kern, &o, (long long)GDB_MAX_GROUPS * WG, &groups, dyn_lds);
+    return launch_groups(ctx, kern, &o, (long long)GDB_MAX_GROUPS * WG, &P->groups, P->dyn_lds);
   });
   if (rc) return rc;
-  const size_t team_ints = (size_t)groups * TW_WORDS * kLine;
-  const size_t bytes = carve_bytes<DevTask>(ntasks) + carve_bytes<int>(team_ints) +
+  P->team_ints = (size_t)P->groups * TW_WORDS * kLine;
+  const size_t bytes = carve_bytes<DevTask>(ntasks) + carve_bytes<int>(P->team_ints) +
                        carve_bytes<int>(ntasks + 64) + carve_bytes<unsigned long long>(8 * (size_t)ntasks) +
                        carve_bytes<int>(scratch_total);
-  rc = ensure_workspace(ctx, bytes);
-  if (rc) return rc;
-  Carve cv(ctx->ws);
-  DevTask* d_tasks = cv.take<DevTask>(ntasks);
-  int* team_state = cv.take<int>(team_ints);
-  int* misc = cv.take<int>(ntasks + 64);
-  unsigned long long* work = cv.take<unsigned long long>(8 * (size_t)ntasks);
+  void* base = nullptr;
+  if (own_memory) {
+    GDB_HIP(hipMalloc(&P->own, bytes));
+    base = P->own;
+  } else {
+    rc = ensure_workspace(ctx, bytes);
+    if (rc) return rc;
+    base = ctx->ws;
+  }
+  Carve cv(base);
+  P->d_tasks = cv.take<DevTask>(ntasks);
+  P->team_state = cv.take<int>(P->team_ints);
+  P->misc = cv.take<int>(ntasks + 64);
+  P->work = cv.take<unsigned long long>(8 * (size_t)ntasks);
   int* scratch = cv.take<int>(scratch_total);
   size_t off = 0;
   for (int t = 0; t < ntasks; ++t) {
@@ -925,55 +948,150 @@ extern "C" int gdb_batch_solve_sel(gdb_ctx* ctx, gdb_task* tasks, int ntasks, in
     dt[t].scratch = scratch + off;
     off += (ints + 63) & ~(size_t)63;
   }
-  GDB_HIP(hipMemcpyAsync(d_tasks, dt.data(), sizeof(DevTask) * ntasks, hipMemcpyHostToDevice, ctx->stream));
-  rc = fill_i32(ctx, team_state, 0, (long long)team_ints);
-  if (!rc) rc = fill_i32(ctx, misc, 0, ntasks + 64);
-  if (!rc) rc = fill_i32(ctx, (int*)work, 0, 16 * (long long)ntasks);
-  if (rc) return rc;
-  int* next_task = misc;
-  int* teams_used = misc + 32;
-  int* results = misc + 64;
-  rc = prepare_launch(ctx);
-  if (rc) return rc;
-  const int G = team_groups > 0 ? team_groups : 1;
-  rc = gdb_dispatch_wgs(wgs, [&](auto W) -> int {
+  GDB_HIP(hipMemcpyAsync(P->d_tasks, dt.data(), sizeof(DevTask) * ntasks, hipMemcpyHostToDevice, ctx->stream));
+  GDB_HIP(hipStreamSynchronize(ctx->stream));  // dt is a local copy
+  return GDB_SUCCESS;
+}
+
+// Clears the team state, counters and work words, and the context's error
+// word (kernels only, so that the sequence can be captured in a graph).
+int enqueue_prep(const BatchPlan& P) {
+  int rc = fill_i32(P.ctx, P.team_state, 0, (long long)P.team_ints);
+  if (!rc) rc = fill_i32(P.ctx, P.misc, 0, P.ntasks + 64);
+  if (!rc) rc = fill_i32(P.ctx, (int*)P.work, 0, 16 * (long long)P.ntasks);
+  if (!rc) rc = prepare_launch(P.ctx);
+  return rc;
+}
+
+int enqueue_kernel(const BatchPlan& P) {
+  gdb_ctx* ctx = P.ctx;
+  int* next_task = P.misc;
+  int* teams_used = P.misc + 32;
+  int* results = P.misc + 64;
+  return gdb_dispatch_wgs(P.wgs, [&](auto W) -> int {
     constexpr int WG = decltype(W)::value;
-    int r2 = timer_start(ctx);
-    if (r2) return r2;
-    if (cfg.mode != GDB_SEL_NONE)
-      hipLaunchKernelGGL((k_batch<WG, true>), dim3(groups), dim3(WG), dyn_lds, ctx->stream, ctx->d, d_tasks,
-                         ntasks, G, team_state, next_task, results, teams_used, work, cfg);
+    if (P.cfg.mode != GDB_SEL_NONE)
+      hipLaunchKernelGGL((k_batch<WG, true>), dim3(P.groups), dim3(WG), P.dyn_lds, ctx->stream, ctx->d,
+                         P.d_tasks, P.ntasks, P.G, P.team_state, next_task, results, teams_used, P.work, P.cfg);
     else
-      hipLaunchKernelGGL((k_batch<WG, false>), dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, d_tasks, ntasks,
-                         G, team_state, next_task, results, teams_used, work, cfg);
+      hipLaunchKernelGGL((k_batch<WG, false>), dim3(P.groups), dim3(WG), 0, ctx->stream, ctx->d, P.d_tasks,
+                         P.ntasks, P.G, P.team_state, next_task, results, teams_used, P.work, P.cfg);
     GDB_HIP(hipGetLastError());
-    return timer_stop(ctx, &st.kernel_ms);
+    return GDB_SUCCESS;
   });
-  if (rc) return rc;
+}
+
+// Copies rounds, work and per-task times back into the caller's tasks.
+int collect_batch(const BatchPlan& P, gdb_task* tasks, gdb_stats* st) {
+  gdb_ctx* ctx = P.ctx;
+  const int ntasks = P.ntasks;
   std::vector<int> hres((size_t)ntasks);
   std::vector<unsigned long long> hwork((size_t)8 * ntasks);
-  GDB_HIP(hipMemcpy(hres.data(), results, sizeof(int) * ntasks, hipMemcpyDeviceToHost));
-  GDB_HIP(hipMemcpy(hwork.data(), work, sizeof(unsigned long long) * 8 * ntasks, hipMemcpyDeviceToHost));
+  GDB_HIP(hipMemcpy(hres.data(), P.misc + 64, sizeof(int) * ntasks, hipMemcpyDeviceToHost));
+  GDB_HIP(hipMemcpy(hwork.data(), P.work, sizeof(unsigned long long) * 8 * ntasks, hipMemcpyDeviceToHost));
   for (int i = 0; i < ntasks; ++i) {
     const unsigned long long* wk = &hwork[(size_t)8 * i];
-    gdb_task& tk = tasks[order[i]];
+    gdb_task& tk = tasks[P.order[i]];
     tk.rounds = hres[i];
-    st.rounds += hres[i];
+    st->rounds += hres[i];
     // Worklist tasks report items / edges processed; colour tasks count the
     // graph once.
     const bool wl = tk.algo != GDB_TASK_COLOR;
-    st.work_items += wl ? (long long)wk[0] : tk.n;
-    st.work_edges += wl ? (long long)wk[1] : tk.m;
+    st->work_items += wl ? (long long)wk[0] : tk.n;
+    st->work_edges += wl ? (long long)wk[1] : tk.m;
     // Device wall clock (100 MHz) from the team's start to end of the task.
     tk.time_us = (double)(wk[3] - wk[2]) / 100.0;
     for (int a = 0; a < 4; ++a) tk.arm_iters[a] = (int)wk[4 + a];
   }
-  st.launches = 1;
-  rc = read_participating(ctx, &st.participating);
+  st->launches = 1;
+  int rc = read_participating(ctx, &st->participating);
   if (rc) return rc;
-  rc = check_device_error(ctx);
+  return check_device_error(ctx);
+}
+
+}  // namespace
+
+extern "C" int gdb_batch_solve_sel(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups,
+                                   const gdb_opts* opts, const gdb_selector* sel, gdb_stats* stats) {
+  if (!ctx || ntasks < 0 || (ntasks > 0 && !tasks) || team_groups < 0) return GDB_E_INVALID;
+  gdb_stats st{};
+  if (ntasks == 0) {
+    if (stats) *stats = st;
+    return GDB_SUCCESS;
+  }
+  BatchPlan P;
+  int rc = plan_batch(ctx, tasks, ntasks, team_groups, opts, sel, false, &P);
+  if (!rc) rc = enqueue_prep(P);
+  if (!rc) rc = timer_start(ctx);
+  if (!rc) rc = enqueue_kernel(P);
+  if (!rc) rc = timer_stop(ctx, &st.kernel_ms);
+  if (rc) return rc;
+  rc = collect_batch(P, tasks, &st);
   if (stats) *stats = st;
   return rc;
+}
+
+// ---- the batch as a captured HIP graph ------------------------------------
+struct gdb_batch_graph {
+  BatchPlan plan;
+  gdb_task* tasks = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int gdb_batch_graph_create(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups,
+                                      const gdb_opts* opts, const gdb_selector* sel, gdb_batch_graph** out) {
+  if (!out || !ctx || ntasks <= 0 || !tasks) return GDB_E_INVALID;
+  *out = nullptr;
+  gdb_batch_graph* g = new gdb_batch_graph();
+  g->tasks = tasks;
+  int rc = plan_batch(ctx, tasks, ntasks, team_groups, opts, sel, true, &g->plan);
+  if (!rc) {
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
+    } else {
+      rc = enqueue_prep(g->plan);
+      if (!rc) rc = enqueue_kernel(g->plan);
+      hipGraph_t graph = nullptr;
+      e = hipStreamEndCapture(ctx->stream, &graph);  // always end the capture
+      if (!rc && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+      g->graph = graph;
+      if (!rc) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+      }
+    }
+  }
+  if (rc) {
+    gdb_batch_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return GDB_SUCCESS;
+}
+
+extern "C" int gdb_batch_graph_launch(gdb_batch_graph* g, gdb_stats* stats) {
+  if (!g || !g->exec) return GDB_E_INVALID;
+  gdb_ctx* ctx = g->plan.ctx;
+  gdb_stats st{};
+  int rc = timer_start(ctx);
+  if (rc) return rc;
+  GDB_HIP(hipGraphLaunch(g->exec, ctx->stream));
+  rc = timer_stop(ctx, &st.kernel_ms);
+  if (rc) return rc;
+  rc = collect_batch(g->plan, g->tasks, &st);
+  if (stats) *stats = st;
+  return rc;
+}
+
+extern "C" int gdb_batch_graph_destroy(gdb_batch_graph* g) {
+  if (!g) return GDB_SUCCESS;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->plan.own) (void)hipFree(g->plan.own);
+  delete g;
+  return GDB_SUCCESS;
 }
 
 extern "C" int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups,

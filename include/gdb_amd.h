@@ -216,7 +216,8 @@ typedef struct {
 } gdb_task;
 
 /* team_groups: groups per team (0 = 1).  opts->wgs and opts->groups pick the
- * launch shape.  stats->rounds is the sum of task rounds. */
+ * launch shape.  Teams take the tasks longest first (estimated (n + m),
+ * SSSP before BFS before colour).  stats->rounds is the sum of task rounds. */
 int gdb_batch_solve(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups, const gdb_opts* opts,
                     gdb_stats* stats);
 
@@ -279,6 +280,18 @@ typedef struct {
  * than NONE need n < 2^30 per task. */
 int gdb_batch_solve_sel(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups,
                         const gdb_opts* opts, const gdb_selector* sel, gdb_stats* stats);
+
+/* The same batch recorded once as a HIP graph (sel may be NULL; the clearing kernels and the
+ * persistent launch) and replayed by gdb_batch_graph_launch; the task array
+ * must stay alive (each replay writes rounds, time_us and arm_iters into
+ * it), and so must the graphs and outputs it points to.  The handle owns its
+ * device memory.  Replays on one context must not overlap. */
+typedef struct gdb_batch_graph gdb_batch_graph;
+int gdb_batch_graph_create(gdb_ctx* ctx, gdb_task* tasks, int ntasks, int team_groups, const gdb_opts* opts,
+                           const gdb_selector* sel, gdb_batch_graph** out);
+int gdb_batch_graph_launch(gdb_batch_graph* g, gdb_stats* stats);
+int gdb_batch_graph_destroy(gdb_batch_graph* g);
+
 
 /* ---- MLP forward / backward / training (device buffers) ----------------- */
 /* y[count][4] = MLP(x[count][16]) with bf16 weights. */
