@@ -178,6 +178,13 @@ def bench_c3(args) -> int:
         b, st = gdb.bc(ctx, dg, dg, sources, gdb.Opts.make(wgs=wgs, groups=groups, variant=variant), out=bc_out)
         return st.kernel_ms, b, st
 
+    if args.engine_only:
+        # Tuned kernels only, at the autotuner's shapes (profiler runs).
+        m_ms = mis(gdb.VARIANT_TUNED, 256, 2048)[0]
+        b_ms = bc(gdb.VARIANT_TUNED, 256, 1024)[0]
+        print(json.dumps({"mis_ms": round(m_ms, 3), "bc_ms": round(b_ms, 3)}), flush=True)
+        ctx.close()
+        return 0
     shapes = [(w, gr) for w in (256, 512, 1024) for gr in (256, 512, 1024, 2048)
               if gr <= gdb.max_resident_groups(w, 0)]
     mt = sorted((mis(gdb.VARIANT_TUNED, w, gr)[0], w, gr) for w, gr in shapes)[0]
@@ -338,7 +345,7 @@ def main() -> int:
     ap.add_argument("--shape", default="",
                     help="c2: wgs,groups,delta,hops,selector (skips the autotuner; selector 0 none, 4 MLP)")
     ap.add_argument("--engine-only", action="store_true",
-                    help="c2: time the batched engine only (profiler runs)")
+                    help="c2 / c3: time the engine's kernels only (profiler runs)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "copy"],
                     help="c2 (default): the headline grid BFS + SSSP; c1 colour R-MAT 14; "
                          "c3 MIS + BC R-MAT 22; c4 4096-graph batch; copy: counter calibration")
