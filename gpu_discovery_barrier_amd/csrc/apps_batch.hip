@@ -93,6 +93,7 @@ __device__ __forceinline__ bool mlp_reuse(const IterState& S, int nin) {
 }
 
 constexpr int kLocalQ = 2048;  // LDS queue capacity per buffer (ints)
+constexpr int kMaxHops = 4096;  // local-step budget limit per team iteration
 
 struct Team {
   gdb_kernel_ctx_t* ctx;
@@ -543,28 +544,93 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
       }
       if (hops > 0 && threadIdx.x < 3) lcnt[threadIdx.x] = 0;
       if (hops > 0) __syncthreads();
-      // Relaxes edge kk from a vertex at distance src_d.  Returns the push
-      // class: 0 none, 1 team queue, 2 far pile, 3 local (LDS) queue.
-      auto relax = [&](int kk, unsigned src_d, int* dst_out) -> int {
-        const int dst = k.col[kk];
-        *dst_out = dst;
-        if ((unsigned)dst >= (unsigned)n) return 0;
-        if (!WEIGHTED) return atomicCAS(&dist[dst], kInf, src_d) == kInf ? 1 : 0;
-        const unsigned wt = UNIT ? 1u : (unsigned)k.w[kk];
-        if (wt >= kInf) return 0;
-        const unsigned cand = src_d + wt;
-        const bool nearp = cand < thresh;
-        if (hops > 0) {
-          const unsigned long long old = atomicMin(&d64[dst], (unsigned long long)cand << 32);
-          if (cand >= (unsigned)(old >> 32)) return 0;
-          if (nearp) return 3;
-          return atomicExch(&fstamp[dst], gen) != gen ? 2 : 0;
+      // Relaxes up to kChunk edges per lane (slot j is edge kk[j] when
+      // ok[j]) from a vertex at distance src_d.  Fills the destinations and
+      // their push classes: 0 none, 1 team queue, 2 far pile, 3 local (LDS)
+      // queue.  The slots are branch-free, so every load is issued before
+      // any atomic and every atomic before any compare: the slots cost one
+      // chain of memory round trips, not kChunk chains.  An unused slot
+      // loads edge 0 (present whenever a slot is used) and sends a no-op
+      // atomic (min with all ones, max with INT_MIN) to this thread's spare
+      // word, a valid vertex that spreads such atomics over many lines.
+      const int spare = (int)(((unsigned)blockIdx.x * blockDim.x + threadIdx.x) % (unsigned)n);
+      auto relax_n = [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], unsigned src_d,
+                         int (&dst)[kChunk], int (&cls)[kChunk]) {
+        unsigned wt[kChunk];
+        bool act[kChunk];
+        unsigned cand[kChunk];
+        int at[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int e = ok[j] ? kk[j] : 0;
+          dst[j] = gdb::ldg(k.col + e);
+          wt[j] = (!WEIGHTED || UNIT) ? 1u : (unsigned)gdb::ldg(k.w + e);
         }
-        const unsigned long long nw = ((unsigned long long)cand << 32) | (nearp ? near_mark : far_mark);
-        const unsigned long long old = atomicMin(&d64[dst], nw);
-        if (cand >= (unsigned)(old >> 32)) return 0;
-        if (nearp) return (unsigned)old == near_mark ? 0 : 1;
-        return atomicExch(&fstamp[dst], gen) != gen ? 2 : 0;
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          act[j] = ok[j] && (unsigned)dst[j] < (unsigned)n && wt[j] < kInf;
+          if (!ok[j]) dst[j] = -1;
+          at[j] = act[j] ? dst[j] : spare;
+          cand[j] = src_d + wt[j];
+          cls[j] = 0;
+        }
+        if constexpr (!WEIGHTED) {
+          // An unused slot compares against all ones, which no distance is.
+          unsigned old[kChunk];
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j)
+            old[j] = gdb::atomic_cas_g(&dist[at[j]], act[j] ? kInf : ~0u, act[j] ? src_d : ~0u);
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) cls[j] = (act[j] && old[j] == kInf) ? 1 : 0;
+        } else {
+          bool imp[kChunk], nearp[kChunk];
+          if (hops > 0) {
+            // The mark word stays 0 with local hops, so a 32-bit atomic on
+            // the distance word does (a 64-bit one has a third of the
+            // throughput).
+            unsigned old[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j)
+              old[j] = gdb::atomic_min_g(reinterpret_cast<unsigned*>(&d64[at[j]]) + 1, act[j] ? cand[j] : ~0u);
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+              imp[j] = act[j] && cand[j] < old[j];
+              nearp[j] = cand[j] < thresh;
+              if (imp[j] && nearp[j]) cls[j] = 3;
+            }
+          } else {
+            unsigned long long old[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+              nearp[j] = cand[j] < thresh;
+              const unsigned long long nw =
+                  ((unsigned long long)cand[j] << 32) | (nearp[j] ? near_mark : far_mark);
+              old[j] = gdb::atomic_min_g(&d64[at[j]], act[j] ? nw : ~0ull);
+            }
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+              imp[j] = act[j] && cand[j] < (unsigned)(old[j] >> 32);
+              if (imp[j] && nearp[j]) cls[j] = (unsigned)old[j] == near_mark ? 0 : 1;
+            }
+          }
+          // An improvement beyond the threshold joins the far pile once per
+          // generation: the first to raise the vertex's stamp to `gen` pushes
+          // it (stamps only grow, and none exceeds gen while relaxing).
+          bool far_any = false;
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) far_any |= imp[j] && !nearp[j];
+          if (gdb::wave_any(far_any)) {
+            int fs[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+              const bool f = imp[j] && !nearp[j];
+              fs[j] = gdb::atomic_max_g(&fstamp[f ? dst[j] : spare], f ? gen : INT_MIN);
+            }
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j)
+              if (imp[j] && !nearp[j]) cls[j] = fs[j] < gen ? 2 : 0;
+          }
+        }
       };
       // Pushes the results of one relaxation step of this wave.  Class 3
       // goes to LDS buffer `lb` when there is room, else to the team queue.
@@ -607,20 +673,28 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
           const int len = (valid && !heavy) ? (e - b) : 0;
           const int maxlen = gdb::wave_max(len);
           for (int c = 0; c < maxlen; c += kChunk) {
-            int dst[kChunk], cls[kChunk];
+            bool ok[kChunk];
+            int kk[kChunk], dst[kChunk], cls[kChunk];
 #pragma unroll
             for (int j = 0; j < kChunk; ++j) {
-              dst[j] = -1;
-              cls[j] = (c + j < len) ? relax(b + c + j, dv, &dst[j]) : 0;
+              ok[j] = c + j < len;
+              kk[j] = b + c + j;
             }
+            relax_n(ok, kk, dv, dst, cls);
             push_n(dst, cls, lb, lc);
           }
           // Heavy rows: the whole wave walks one row, 64 edges per step.
           gdb::wave_coop_foreach(
               heavy, b, e, dv,
-              [&](int kk, bool ok, unsigned src_d) {
-                int dst[kChunk] = {-1, -1, -1, -1}, cls[kChunk] = {0, 0, 0, 0};
-                cls[0] = ok ? relax(kk, src_d, &dst[0]) : 0;
+              [&](int e, bool ok, unsigned src_d) {
+                bool oks[kChunk];
+                int kk[kChunk], dst[kChunk], cls[kChunk];
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                  oks[j] = j == 0 && ok;
+                  kk[j] = e;
+                }
+                relax_n(oks, kk, src_d, dst, cls);
                 push_n(dst, cls, lb, lc);
               },
               -1);
@@ -908,7 +982,7 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
         (a.n > 0 && !a.row) || (a.m > 0 && !a.col) || (a.algo == GDB_TASK_SSSP && a.m > 0 && !a.w) ||
         (a.algo == GDB_TASK_COLOR && a.n > 0 && !a.node_value) ||
         (a.algo != GDB_TASK_COLOR && a.n > 0 && (a.source < 0 || a.source >= a.n)) || a.delta < 0 ||
-        a.hops < 0 || a.hops > 64 || (a.trow && !a.tcol) ||
+        a.hops < 0 || a.hops > kMaxHops || (a.trow && !a.tcol) ||
         (cfg.mode != GDB_SEL_NONE && a.algo != GDB_TASK_COLOR && a.n > kNinMask))
       return GDB_E_INVALID;
     const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : 2) * a.n;

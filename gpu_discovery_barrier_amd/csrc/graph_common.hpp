@@ -13,6 +13,31 @@ __device__ __forceinline__ unsigned shfl_xor(unsigned x, int o) {
   return (unsigned)__shfl_xor((int)x, o, 64);
 }
 
+// Global-address-space accessors.  Graph arrays reach the kernels through
+// structs, so the compiler sees generic pointers and emits flat instructions,
+// whose waits also cover LDS traffic; these keep the accesses global_*.
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+__device__ __forceinline__ unsigned atomic_min_g(unsigned* p, unsigned v) {
+  return __hip_atomic_fetch_min((__attribute__((address_space(1))) unsigned*)p, v, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long atomic_min_g(unsigned long long* p, unsigned long long v) {
+  return __hip_atomic_fetch_min((__attribute__((address_space(1))) unsigned long long*)p, v, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int atomic_max_g(int* p, int v) {
+  return __hip_atomic_fetch_max((__attribute__((address_space(1))) int*)p, v, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned atomic_cas_g(unsigned* p, unsigned expected, unsigned desired) {
+  __hip_atomic_compare_exchange_strong((__attribute__((address_space(1))) unsigned*)p, &expected, desired,
+                                       __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return expected;
+}
+
 struct MaxFound {  // running max plus "at least one candidate seen"
   float mx;
   int found;

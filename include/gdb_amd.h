@@ -204,7 +204,8 @@ typedef struct {
   void* out;                /* device output: u32 distances or i32 colours */
   int delta;                /* SSSP near-far band width (0 = plain worklist) */
   int hops;                 /* BFS / SSSP: local hops a group may run past its share of
-                               an iteration before the team barrier (0 = level-synchronous) */
+                               an iteration before the team barrier, 0..4096
+                               (0 = level-synchronous) */
   const int* trow;          /* in-edge CSR (the transpose; the CSR itself for a symmetric
                                graph) for pull iterations; NULL = push only */
   const int* tcol;
