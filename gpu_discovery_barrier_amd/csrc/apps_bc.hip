@@ -33,7 +33,7 @@
 namespace {
 
 constexpr int kSerialLimit = 16;
-constexpr int kChunk = 4;  // edges a lane handles per step in the tuned BFS walk
+constexpr int kChunk = 4;  // edge slots a lane handles per step in the tuned walks
 
 __device__ __forceinline__ void fatomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(WGS) void k_bc_plain(gdb_kernel_ctx_t* ctx, const i
 // forward pass, then of coef = (1 + delta) / sigma once the vertex's delta
 // is final in the backward pass (delta itself is only needed there).
 __device__ __forceinline__ int2 ld_state(const int2* ds, int v) {
-  const long long x = *(const long long*)&ds[v];
+  const long long x = gdb::ldg((const long long*)&ds[v]);
   return make_int2((int)(x & 0xffffffffll), (int)(x >> 32));
 }
 
@@ -138,6 +138,10 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
   // A symmetric graph is its own transpose: one walk over a frontier
   // vertex's edges both sums sigma from level l-1 and claims level l+1.
   const bool sym = row == row_t && col == col_t;
+  // Unused edge slots read this thread's spare vertex and send it a no-op
+  // compare-and-swap (no level is INT_MIN), which keeps the slots
+  // branch-free and spreads such accesses over many lines.
+  const int spare = n > 0 ? (int)((unsigned)tid % (unsigned)n) : 0;
   int nbar = 0, levels = 0;
   for (int i = tid; i < n; i += stride) {
     bc[i] = 0.f;
@@ -178,48 +182,40 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
           b = row[v];
           e = row[v + 1];
         }
-        // Walks v's out-edges (light rows four edges per lane step, so that
-        // four gathers and claims are in flight and one wave push serves
-        // them; longer rows by the whole wave): claims unreached targets for
-        // level l+1 and, when `pull_sigma` (a symmetric graph), sums sigma
-        // over the targets on level l-1.
+        // Walks v's out-edges kChunk slots per lane step (light rows by their
+        // own lane, long rows by the whole wave): claims unreached targets
+        // for level l+1 and, when `pull_sigma` (a symmetric graph), sums
+        // sigma over the targets on level l-1.  Per step the slots' column
+        // loads, then their state gathers, then their claims are issued
+        // together; one wave push serves the step.
         auto walk = [&](bool pull_sigma) -> float {
-          const bool heavy = valid && (e - b) > kSerialLimit;
-          const int len = (valid && !heavy) ? (e - b) : 0;
-          const int maxlen = gdb::wave_max(len);
-          float acc = 0.f;
-          for (int c = 0; c < maxlen; c += kChunk) {
-            int w[kChunk];
-            bool push[kChunk];
+          return gdb::wave_walk_n<kChunk>(
+              valid != 0, b, e, 0.f,
+              [&](const bool (&ok)[kChunk], const int (&kk)[kChunk]) -> float {
+                int w[kChunk], old[kChunk];
+                int2 x[kChunk];
+                bool in[kChunk], claim[kChunk], push[kChunk];
 #pragma unroll
-            for (int j = 0; j < kChunk; ++j) w[j] = c + j < len ? col[b + c + j] : -1;
+                for (int j = 0; j < kChunk; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
 #pragma unroll
-            for (int j = 0; j < kChunk; ++j) {
-              push[j] = false;
-              if ((unsigned)w[j] < (unsigned)n) {
-                const int2 x = ld_state(ds, w[j]);
-                if (pull_sigma && x.x == l - 1) acc += __int_as_float(x.y);
-                push[j] = x.x == -1 && atomicCAS(&ds[w[j]].x, -1, l + 1) == -1;
-              }
-            }
-            gdb::wave_push_n<kChunk>(push, w, q, tail);
-          }
-          const float hv = gdb::wave_coop_walk(
-              heavy, b, e, 0.f,
-              [&](int k, bool ok) -> float {
-                int w = -1;
-                int2 x = make_int2(0, 0);
-                if (ok) {
-                  w = col[k];
-                  ok = (unsigned)w < (unsigned)n;
-                  if (ok) x = ld_state(ds, w);
+                for (int j = 0; j < kChunk; ++j) {
+                  in[j] = ok[j] && (unsigned)w[j] < (unsigned)n;
+                  x[j] = ld_state(ds, in[j] ? w[j] : spare);
                 }
-                const bool push = ok && x.x == -1 && atomicCAS(&ds[w].x, -1, l + 1) == -1;
-                gdb::wave_push(push, w, q, tail);
-                return (ok && pull_sigma && x.x == l - 1) ? __int_as_float(x.y) : 0.f;
+                float acc = 0.f;
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                  claim[j] = in[j] && x[j].x == -1;
+                  old[j] = gdb::atomic_cas_g(&ds[claim[j] ? w[j] : spare].x, claim[j] ? -1 : INT_MIN,
+                                             claim[j] ? l + 1 : INT_MIN);
+                  if (pull_sigma && in[j] && x[j].x == l - 1) acc += __int_as_float(x[j].y);
+                }
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) push[j] = claim[j] && old[j] == -1;
+                gdb::wave_push_n<kChunk>(push, w, q, tail);
+                return acc;
               },
-              plus, -1);
-          return heavy ? hv : acc;
+              plus, kSerialLimit);
         };
         float sg = 0.f;
         if (sym) {
@@ -230,13 +226,21 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
             bt = row_t[v];
             et = row_t[v + 1];
           }
-          sg = gdb::wave_coop_reduce(
+          sg = gdb::wave_walk_n<kChunk>(
               valid != 0 && l > 0, bt, et, 0.f,
-              [&](int k) -> float {
-                const int u = col_t[k];
-                if ((unsigned)u >= (unsigned)n) return 0.f;
-                const int2 x = ld_state(ds, u);
-                return x.x == l - 1 ? __int_as_float(x.y) : 0.f;
+              [&](const bool (&ok)[kChunk], const int (&kk)[kChunk]) -> float {
+                int u[kChunk];
+                int2 x[kChunk];
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) u[j] = gdb::ldg(col_t + (ok[j] ? kk[j] : 0));
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j)
+                  x[j] = ld_state(ds, ok[j] && (unsigned)u[j] < (unsigned)n ? u[j] : spare);
+                float acc = 0.f;
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j)
+                  if (ok[j] && (unsigned)u[j] < (unsigned)n && x[j].x == l - 1) acc += __int_as_float(x[j].y);
+                return acc;
               },
               plus, kSerialLimit);
           walk(false);
@@ -293,13 +297,21 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
           b = row[w];
           e = row[w + 1];
         }
-        const float acc = gdb::wave_coop_reduce(
+        const float acc = gdb::wave_walk_n<kChunk>(
             valid != 0, b, e, 0.f,
-            [&](int k) -> float {
-              const int v = col[k];
-              if ((unsigned)v >= (unsigned)n) return 0.f;
-              const int2 x = ld_state(ds, v);
-              return x.x == ll + 1 ? __int_as_float(x.y) : 0.f;
+            [&](const bool (&ok)[kChunk], const int (&kk)[kChunk]) -> float {
+              int v[kChunk];
+              int2 x[kChunk];
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j) v[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j)
+                x[j] = ld_state(ds, ok[j] && (unsigned)v[j] < (unsigned)n ? v[j] : spare);
+              float a = 0.f;
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j)
+                if (ok[j] && (unsigned)v[j] < (unsigned)n && x[j].x == ll + 1) a += __int_as_float(x[j].y);
+              return a;
             },
             plus, kSerialLimit);
         if (valid) {

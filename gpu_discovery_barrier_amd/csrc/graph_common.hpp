@@ -37,6 +37,11 @@ __device__ __forceinline__ unsigned atomic_cas_g(unsigned* p, unsigned expected,
                                        __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return expected;
 }
+__device__ __forceinline__ int atomic_cas_g(int* p, int expected, int desired) {
+  __hip_atomic_compare_exchange_strong((__attribute__((address_space(1))) int*)p, &expected, desired,
+                                       __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return expected;
+}
 
 struct MaxFound {  // running max plus "at least one candidate seen"
   float mx;
@@ -226,6 +231,57 @@ __device__ __forceinline__ int wave_max(int v) {
 
 // Wave-wide OR of a predicate.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// Chunked cooperative walk over the edge ranges [b, e) of the lanes of one
+// wave: step(ok, kk) handles N edge slots of the calling lane at once (slot j
+// is edge kk[j] when ok[j]) and returns the lane's contribution; it is called
+// in wave-uniform steps, so it may use wave collectives, and it should issue
+// the slots' loads and atomics branch-free, all loads before any use, so that
+// the N slots cost one chain of memory round trips.  Short ranges are walked
+// by their own lane N edges per step; long ranges by the whole wave, N x 64
+// consecutive edges per step (slot j of lane i takes edge base + i + 64 j, so
+// each slot's loads are coalesced), and reduced into the owner lane.
+template <int N, typename T, typename Step, typename R>
+__device__ __forceinline__ T wave_walk_n(bool active, int b, int e, T identity, Step step, R red,
+                                         int serial_limit) {
+  const bool heavy = active && (e - b) > serial_limit;
+  const int len = (active && !heavy) ? (e - b) : 0;
+  const int maxlen = wave_max(len);
+  T acc = identity;
+  for (int c = 0; c < maxlen; c += N) {
+    bool ok[N];
+    int kk[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      ok[j] = c + j < len;
+      kk[j] = b + c + j;
+    }
+    acc = red(acc, step(ok, kk));
+  }
+  unsigned long long mask = __ballot(heavy);
+  const int lane = lane_id();
+  while (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    const int lb = __shfl(b, leader, 64);
+    const int le = __shfl(e, leader, 64);
+    T part = identity;
+    for (int base = lb; base < le; base += 64 * N) {
+      bool ok[N];
+      int kk[N];
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        kk[j] = base + lane + 64 * j;
+        ok[j] = kk[j] < le;
+      }
+      part = red(part, step(ok, kk));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part = red(part, shfl_xor(part, off));
+    if (lane == leader) acc = part;
+    mask &= mask - 1;
+  }
+  return acc;
+}
 
 // Uniform-trip grid-stride loop over [0, n): every thread of a group runs
 // the same number of iterations (i may be >= n in the last one), so the body
