@@ -556,6 +556,17 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
       const int spare = (int)(((unsigned)blockIdx.x * blockDim.x + threadIdx.x) % (unsigned)n);
       auto relax_n = [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], unsigned src_d,
                          int (&dst)[kChunk], int (&cls)[kChunk]) {
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          dst[j] = -1;
+          cls[j] = 0;
+        }
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) any |= ok[j];
+        // Lanes without a slot this step stay out (no memory traffic); inside
+        // the branch every result is consumed before it closes.
+        if (!any) return;
         unsigned wt[kChunk];
         bool act[kChunk];
         unsigned cand[kChunk];
@@ -619,7 +630,7 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
           bool far_any = false;
 #pragma unroll
           for (int j = 0; j < kChunk; ++j) far_any |= imp[j] && !nearp[j];
-          if (gdb::wave_any(far_any)) {
+          if (far_any) {
             int fs[kChunk];
 #pragma unroll
             for (int j = 0; j < kChunk; ++j) {
@@ -859,8 +870,11 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
   return r;
 }
 
+// 512-thread groups serve many small tasks at once (throughput-bound): they
+// are held to 80 VGPRs, three groups per CU instead of two, at the price of a
+// few spilled long-lived values.
 template <int WGS, bool SEL>
-__global__ __launch_bounds__(WGS) void k_batch(gdb_kernel_ctx_t* ctx, const DevTask* __restrict__ tasks,
+__global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(WGS == 512 && !SEL ? 6 : 1))) void k_batch(gdb_kernel_ctx_t* ctx, const DevTask* __restrict__ tasks,
                                                int ntasks, int G, int* team_state, int* next_task,
                                                int* results, int* teams_used,
                                                unsigned long long* work, gdbsel::Cfg sel) {

@@ -192,26 +192,37 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
           return gdb::wave_walk_n<kChunk>(
               valid != 0, b, e, 0.f,
               [&](const bool (&ok)[kChunk], const int (&kk)[kChunk]) -> float {
-                int w[kChunk], old[kChunk];
-                int2 x[kChunk];
-                bool in[kChunk], claim[kChunk], push[kChunk];
-#pragma unroll
-                for (int j = 0; j < kChunk; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
-#pragma unroll
-                for (int j = 0; j < kChunk; ++j) {
-                  in[j] = ok[j] && (unsigned)w[j] < (unsigned)n;
-                  x[j] = ld_state(ds, in[j] ? w[j] : spare);
-                }
+                int w[kChunk];
+                bool push[kChunk];
                 float acc = 0.f;
+                bool any = false;
 #pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
-                  claim[j] = in[j] && x[j].x == -1;
-                  old[j] = gdb::atomic_cas_g(&ds[claim[j] ? w[j] : spare].x, claim[j] ? -1 : INT_MIN,
-                                             claim[j] ? l + 1 : INT_MIN);
-                  if (pull_sigma && in[j] && x[j].x == l - 1) acc += __int_as_float(x[j].y);
+                  w[j] = -1;
+                  push[j] = false;
+                  any |= ok[j];
                 }
+                if (any) {  // lanes without a slot this step stay out
+                  int old[kChunk];
+                  int2 x[kChunk];
+                  bool in[kChunk], claim[kChunk];
 #pragma unroll
-                for (int j = 0; j < kChunk; ++j) push[j] = claim[j] && old[j] == -1;
+                  for (int j = 0; j < kChunk; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
+#pragma unroll
+                  for (int j = 0; j < kChunk; ++j) {
+                    in[j] = ok[j] && (unsigned)w[j] < (unsigned)n;
+                    x[j] = ld_state(ds, in[j] ? w[j] : spare);
+                  }
+#pragma unroll
+                  for (int j = 0; j < kChunk; ++j) {
+                    claim[j] = in[j] && x[j].x == -1;
+                    old[j] = gdb::atomic_cas_g(&ds[claim[j] ? w[j] : spare].x, claim[j] ? -1 : INT_MIN,
+                                               claim[j] ? l + 1 : INT_MIN);
+                    if (pull_sigma && in[j] && x[j].x == l - 1) acc += __int_as_float(x[j].y);
+                  }
+#pragma unroll
+                  for (int j = 0; j < kChunk; ++j) push[j] = claim[j] && old[j] == -1;
+                }
                 gdb::wave_push_n<kChunk>(push, w, q, tail);
                 return acc;
               },

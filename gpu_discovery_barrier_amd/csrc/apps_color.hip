@@ -95,6 +95,8 @@ __device__ __forceinline__ int fkey(float f) {
 
 constexpr int kHeavyDeg = 16;    // rows longer than this are split into chunks
 constexpr int kChunkEdges = 64;  // edges of a heavy row per chunk (one per lane)
+constexpr int kSlots = 8;        // edges a lane gathers per step on light rows
+constexpr int kChunksPerStep = 4;  // heavy-row chunks a wave handles per step
 
 template <int WGS>
 __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
@@ -109,6 +111,9 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
   const int lane = gdb::lane_id();
   const int wave = tid >> 6, nwaves = stride >> 6;
   const int none = fkey(-1.f);
+  // Unused gather slots read this thread's spare vertex, so that every
+  // slot's loads are issued branch-free, before any of them is used.
+  const int spare = n > 0 ? (int)((unsigned)tid % (unsigned)n) : 0;
   int r = 1;
   int nbar = 0;
   for (;;) {
@@ -121,8 +126,8 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
     if (tid == 0) gdb::st_rlx(&stops[(r + 1) % 3], 0);
     for (int h = tid; h < nheavy; h += stride) md_next[heavy[h]] = none;
     // u counts as uncoloured in round r unless coloured before round r-1 or
-    // by round r-1 itself (node_value[u] > max_d_{r-1}[u]).
-    auto uncoloured = [&](int u) { return color[u] == -1 && !(have_prev && fkey(nv[u]) > md_prev[u]); };
+    // by round r-1 itself (node_value[u] > max_d_{r-1}[u]): below, color[u]
+    // == -1 && !(have_prev && fkey(nv[u]) > md_prev[u]).
     bool st = false;
     GDB_FOR_UNIFORM(v, valid, tid, stride, n) {
       bool active = false;
@@ -143,44 +148,93 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
         }
       }
       const bool deg1 = (e - b) == 1;
-      auto edge = [&](int k) -> gdb::MaxFound {
-        const int u = col[k];
-        return uncoloured(u) ? gdb::MaxFound{nv[u], 1} : gdb::MaxFound{-1.f, 0};
+      // kSlots edges per lane step: the column loads, then the three
+      // gathers per neighbour, then the compares.
+      auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> gdb::MaxFound {
+        int u[kSlots], cu[kSlots], mu[kSlots];
+        float xu[kSlots];
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) u[j] = ok[j] ? gdb::ldg(col + kk[j]) : spare;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+          const int uu = ok[j] ? u[j] : spare;
+          cu[j] = gdb::ldg(color + uu);
+          xu[j] = gdb::ldg(nv + uu);
+          mu[j] = gdb::ldg(md_prev + uu);
+        }
+        gdb::MaxFound a{-1.f, 0};
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+          const bool unc = ok[j] && cu[j] == -1 && !(have_prev && fkey(xu[j]) > mu[j]);
+          if (unc) {
+            a.mx = xu[j] > a.mx ? xu[j] : a.mx;
+            a.found = 1;
+          }
+        }
+        return a;
       };
       auto red = [](gdb::MaxFound a, gdb::MaxFound c) {
         return gdb::MaxFound{a.mx > c.mx ? a.mx : c.mx, a.found | c.found};
       };
       const gdb::MaxFound res =
-          gdb::wave_coop_reduce(active && !deg1, b, e, gdb::MaxFound{-1.f, 0}, edge, red, kSerialLimit);
+          gdb::wave_walk_n<kSlots>(active && !deg1, b, e, gdb::MaxFound{-1.f, 0}, step, red, kHeavyDeg);
       if (active) {
         md_cur[v] = deg1 ? none : fkey(res.mx);
         if (!deg1 && res.found) st = true;
       }
     }
-    // Long rows, one chunk of kChunkEdges edges per wave step, spread over
-    // every wave of the grid; the chunks of one row meet in md_cur[v] by
+    // Long rows in chunks of kChunkEdges edges (one per lane), spread over
+    // every wave of the grid, kChunksPerStep chunks per wave step with their
+    // loads issued together; the chunks of one row meet in md_cur[v] by
     // atomicMax.
-    for (int c = wave; c < nchunks; c += nwaves) {
-      const int4 ch = chunks[c];  // (row, first edge, end edge)
-      const int v = ch.x;
-      const int k = ch.y + lane;
-      // v's state and the chunk's edge are loaded together.
-      const int u = k < ch.z ? col[k] : -1;
-      if (color[v] != -1 || (have_prev && fkey(nv[v]) > md_prev[v])) continue;  // wave-uniform
-      float mx = -1.f;
-      int found = 0;
-      if (u >= 0 && uncoloured(u)) {
-        found = 1;
-        mx = nv[u];
+    for (int c0 = wave; c0 < nchunks; c0 += kChunksPerStep * nwaves) {
+      int4 ch[kChunksPerStep];
+      bool has[kChunksPerStep];
+#pragma unroll
+      for (int j = 0; j < kChunksPerStep; ++j) {
+        const int c = c0 + j * nwaves;
+        has[j] = c < nchunks;
+        ch[j] = chunks[has[j] ? c : c0];  // (row, first edge, end edge)
+      }
+      int u[kChunksPerStep], cv[kChunksPerStep], mv[kChunksPerStep];
+      float xv[kChunksPerStep];
+#pragma unroll
+      for (int j = 0; j < kChunksPerStep; ++j) {
+        const int k = ch[j].y + lane;
+        u[j] = gdb::ldg(col + (k < ch[j].z ? k : ch[j].y));
+        if (k >= ch[j].z) u[j] = -1;
+        cv[j] = gdb::ldg(color + ch[j].x);
+        xv[j] = gdb::ldg(nv + ch[j].x);
+        mv[j] = gdb::ldg(md_prev + ch[j].x);
+      }
+      int cu[kChunksPerStep], mu[kChunksPerStep];
+      float xu[kChunksPerStep];
+#pragma unroll
+      for (int j = 0; j < kChunksPerStep; ++j) {
+        const int uu = u[j] >= 0 ? u[j] : spare;
+        cu[j] = gdb::ldg(color + uu);
+        xu[j] = gdb::ldg(nv + uu);
+        mu[j] = gdb::ldg(md_prev + uu);
       }
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-        found |= __shfl_xor(found, off, 64);
-      }
-      if (found) {
-        st = true;
-        if (lane == 0) atomicMax(&md_cur[v], fkey(mx));
+      for (int j = 0; j < kChunksPerStep; ++j) {
+        // Skip a chunk whose row is coloured (wave-uniform).
+        if (!has[j] || cv[j] != -1 || (have_prev && fkey(xv[j]) > mv[j])) continue;
+        float mx = -1.f;
+        int found = 0;
+        if (u[j] >= 0 && cu[j] == -1 && !(have_prev && fkey(xu[j]) > mu[j])) {
+          found = 1;
+          mx = xu[j];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+          found |= __shfl_xor(found, off, 64);
+        }
+        if (found) {
+          st = true;
+          if (lane == 0) atomicMax(&md_cur[ch[j].x], fkey(mx));
+        }
       }
     }
     if (gdb::wave_any(st) && lane == 0) gdb::st_rlx(&stops[r % 3], 1);
