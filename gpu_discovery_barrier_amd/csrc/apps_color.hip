@@ -104,7 +104,8 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
                                                      const float* __restrict__ nv, int* color,
                                                      int* stops, int* mdk, const int* __restrict__ heavy,
                                                      int nheavy, const int4* __restrict__ chunks,
-                                                     int nchunks, int n, int max_rounds, int* out) {
+                                                     int nchunks, int4* clist, int n, int max_rounds,
+                                                     int* out) {
   DISCOVERY_PROTOCOL(ctx);
   const int tid = p_get_global_id(ctx, &local_ctx);
   const int stride = p_get_global_size(ctx, &local_ctx);
@@ -114,6 +115,10 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
   // Unused gather slots read this thread's spare vertex, so that every
   // slot's loads are issued branch-free, before any of them is used.
   const int spare = n > 0 ? (int)((unsigned)tid % (unsigned)n) : 0;
+  // Heavy-row chunks this wave still walks: its share of the chunk table in
+  // round 1 (chunks wave + k * nwaves), then the ones whose rows were still
+  // uncoloured, compacted in place into the same strided slots of clist.
+  int wchunks = nchunks > wave ? (nchunks - wave + nwaves - 1) / nwaves : 0;
   int r = 1;
   int nbar = 0;
   for (;;) {
@@ -154,7 +159,7 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
         int u[kSlots], cu[kSlots], mu[kSlots];
         float xu[kSlots];
 #pragma unroll
-        for (int j = 0; j < kSlots; ++j) u[j] = ok[j] ? gdb::ldg(col + kk[j]) : spare;
+        for (int j = 0; j < kSlots; ++j) u[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));  // edge 0 exists here
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) {
           const int uu = ok[j] ? u[j] : spare;
@@ -186,15 +191,19 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
     // Long rows in chunks of kChunkEdges edges (one per lane), spread over
     // every wave of the grid, kChunksPerStep chunks per wave step with their
     // loads issued together; the chunks of one row meet in md_cur[v] by
-    // atomicMax.
-    for (int c0 = wave; c0 < nchunks; c0 += kChunksPerStep * nwaves) {
+    // atomicMax.  A wave keeps only the chunks of rows still uncoloured (a
+    // row coloured by round r is dropped in round r+1), so the walk shrinks
+    // as rows are coloured.  Compaction in place is safe: a step's loads
+    // complete before its stores, which only reach slots already read.
+    const int4* cin = r == 1 ? chunks : clist;
+    int kept = 0;
+    for (int k0 = 0; k0 < wchunks; k0 += kChunksPerStep) {
       int4 ch[kChunksPerStep];
-      bool has[kChunksPerStep];
+      bool has[kChunksPerStep], keep[kChunksPerStep];
 #pragma unroll
       for (int j = 0; j < kChunksPerStep; ++j) {
-        const int c = c0 + j * nwaves;
-        has[j] = c < nchunks;
-        ch[j] = chunks[has[j] ? c : c0];  // (row, first edge, end edge)
+        has[j] = k0 + j < wchunks;
+        ch[j] = cin[wave + (size_t)(has[j] ? k0 + j : k0) * nwaves];  // (row, first edge, end edge)
       }
       int u[kChunksPerStep], cv[kChunksPerStep], mv[kChunksPerStep];
       float xv[kChunksPerStep];
@@ -218,25 +227,29 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
       }
 #pragma unroll
       for (int j = 0; j < kChunksPerStep; ++j) {
-        // Skip a chunk whose row is coloured (wave-uniform).
-        if (!has[j] || cv[j] != -1 || (have_prev && fkey(xv[j]) > mv[j])) continue;
-        float mx = -1.f;
-        int found = 0;
-        if (u[j] >= 0 && cu[j] == -1 && !(have_prev && fkey(xu[j]) > mu[j])) {
-          found = 1;
-          mx = xu[j];
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-          found |= __shfl_xor(found, off, 64);
-        }
-        if (found) {
-          st = true;
-          if (lane == 0) atomicMax(&md_cur[ch[j].x], fkey(mx));
+        keep[j] = has[j] && cv[j] == -1 && !(have_prev && fkey(xv[j]) > mv[j]);  // wave-uniform
+        if (keep[j]) {
+          if (lane == 0) clist[wave + (size_t)kept * nwaves] = ch[j];
+          ++kept;
         }
       }
+      // Each chunk's maximum key over its uncoloured neighbours (INT_MIN:
+      // none; only a NaN priority could have that key), the four wave reductions
+      // independent of each other.
+      int key[kChunksPerStep];
+#pragma unroll
+      for (int j = 0; j < kChunksPerStep; ++j) {
+        const bool unc = u[j] >= 0 && cu[j] == -1 && !(have_prev && fkey(xu[j]) > mu[j]);
+        key[j] = gdb::wave_max(unc ? fkey(xu[j]) : INT_MIN);
+      }
+#pragma unroll
+      for (int j = 0; j < kChunksPerStep; ++j)
+        if (keep[j] && key[j] != INT_MIN) {  // wave-uniform
+          st = true;
+          if (lane == 0) atomicMax(&md_cur[ch[j].x], key[j]);
+        }
     }
+    wchunks = kept;
     if (gdb::wave_any(st) && lane == 0) gdb::st_rlx(&stops[r % 3], 1);
     if (!discovery_barrier(ctx, &local_ctx)) return;
     ++nbar;
@@ -352,7 +365,7 @@ extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* 
   const int nchunks = (int)chunks.size();
   const size_t bytes = carve_bytes<float>((size_t)2 * n) + carve_bytes<int>(16) +
                        (tuned_persistent ? carve_bytes<int>((size_t)3 * n) + carve_bytes<int>(nheavy + 1) +
-                                               carve_bytes<int4>(nchunks + 1)
+                                               carve_bytes<int4>(nchunks + 1) * 2
                                          : 0);
   int rc = ensure_workspace(ctx, bytes);
   if (rc) return rc;
@@ -362,6 +375,7 @@ extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* 
   int* mdk = tuned_persistent ? cv.take<int>((size_t)3 * n) : nullptr;
   int* d_heavy = tuned_persistent ? cv.take<int>(nheavy + 1) : nullptr;
   int4* d_chunks = tuned_persistent ? cv.take<int4>(nchunks + 1) : nullptr;
+  int4* d_clist = tuned_persistent ? cv.take<int4>(nchunks + 1) : nullptr;
   if (nheavy) {
     GDB_HIP(hipMemcpyAsync(d_heavy, heavy.data(), sizeof(int) * nheavy, hipMemcpyHostToDevice, ctx->stream));
     GDB_HIP(hipMemcpyAsync(d_chunks, chunks.data(), sizeof(int4) * nchunks, hipMemcpyHostToDevice,
@@ -421,8 +435,8 @@ extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* 
       if (r2) return r2;
       if (tuned)
         hipLaunchKernelGGL(k_color_tuned<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row,
-                           col, node_value, color, flags, mdk, d_heavy, nheavy, d_chunks, nchunks, n,
-                           max_rounds, out);
+                           col, node_value, color, flags, mdk, d_heavy, nheavy, d_chunks, nchunks,
+                           d_clist, n, max_rounds, out);
       else
         hipLaunchKernelGGL(k_color_plain<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row,
                            col, node_value, color, flags, flags + 4, maxd, n, max_rounds, out);

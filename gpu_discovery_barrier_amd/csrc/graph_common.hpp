@@ -220,14 +220,10 @@ __device__ __forceinline__ void wave_push_n(const bool (&push)[N], const int (&i
   }
 }
 
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
-}
+// Wave-wide max over the active lanes, by DPP row shifts (no LDS round
+// trips), from the device library.
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
 
 // Wave-wide OR of a predicate.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
