@@ -647,17 +647,29 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
       // goes to LDS buffer `lb` when there is room, else to the team queue.
       auto push_n = [&](const int (&dst)[kChunk], const int (&cls)[kChunk], int lb, int* lc) {
         bool to_team[kChunk], to_far[kChunk];
+        unsigned long long m3[kChunk];
+        int n3 = 0;
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
-          const int c = cls[j];
-          bool team_push = c == 1;
-          if (c == 3) {
-            const int pos = atomicAdd(lc, 1);
-            if (pos < kLocalQ) lq[lb][pos] = dst[j];
-            else team_push = atomicExch(&gstamp[dst[j]], it) != it;
+          to_team[j] = cls[j] == 1;
+          to_far[j] = cls[j] == 2;
+          m3[j] = __ballot(cls[j] == 3);
+          n3 += __popcll(m3[j]);
+        }
+        if (n3) {  // one LDS atomic per wave for the step's local pushes
+          int pos = 0;
+          if (gdb::lane_id() == 0) pos = atomicAdd(lc, n3);
+          pos = gdb::bcast(pos, 0);
+          const unsigned long long lt = (1ull << gdb::lane_id()) - 1ull;
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) {
+            if (cls[j] == 3) {
+              const int at = pos + __popcll(m3[j] & lt);
+              if (at < kLocalQ) lq[lb][at] = dst[j];
+              else to_team[j] = atomicExch(&gstamp[dst[j]], it) != it;
+            }
+            pos += __popcll(m3[j]);
           }
-          to_team[j] = team_push;
-          to_far[j] = c == 2;
         }
         gdb::wave_push_n<kChunk>(to_team, dst, outq, out_tail);
         if (near_far) gdb::wave_push_n<kChunk>(to_far, dst, fpile, ftail);

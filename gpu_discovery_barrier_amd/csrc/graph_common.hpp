@@ -7,6 +7,15 @@ namespace gdb {
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Value of `v` in lane `src` (wave-uniform) for every lane: a scalar read of
+// the register, no LDS round trip.
+__device__ __forceinline__ int bcast(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+
+// Wave-wide max over the active lanes, by DPP row shifts (no LDS round
+// trips), from the device library.
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
+
 __device__ __forceinline__ float shfl_xor(float x, int o) { return __shfl_xor(x, o, 64); }
 __device__ __forceinline__ int shfl_xor(int x, int o) { return __shfl_xor(x, o, 64); }
 __device__ __forceinline__ unsigned shfl_xor(unsigned x, int o) {
@@ -71,8 +80,8 @@ __device__ __forceinline__ T wave_coop_reduce(bool active, int b, int e, T ident
   const int lane = lane_id();
   while (mask) {
     const int leader = __ffsll((long long)mask) - 1;
-    const int lb = __shfl(b, leader, 64);
-    const int le = __shfl(e, leader, 64);
+    const int lb = bcast(b, leader);
+    const int le = bcast(e, leader);
     T part = identity;
 #pragma unroll 4
     for (int k = lb + lane; k < le; k += 64) part = red(part, edge(k));
@@ -97,20 +106,15 @@ __device__ __forceinline__ void wave_coop_foreach(bool active, int b, int e, uns
                                                   int serial_limit) {
   const bool heavy = active && (e - b) > serial_limit;
   int len = (active && !heavy) ? (e - b) : 0;
-  int maxlen = len;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int o = __shfl_xor(maxlen, off, 64);
-    maxlen = o > maxlen ? o : maxlen;
-  }
+  const int maxlen = wave_max(len);
   for (int t = 0; t < maxlen; ++t) fn(b + t, t < len, payload);
   unsigned long long mask = __ballot(heavy);
   const int lane = lane_id();
   while (mask) {
     const int leader = __ffsll((long long)mask) - 1;
-    const int lb = __shfl(b, leader, 64);
-    const int le = __shfl(e, leader, 64);
-    const unsigned lp = (unsigned)__shfl((int)payload, leader, 64);
+    const int lb = bcast(b, leader);
+    const int le = bcast(e, leader);
+    const unsigned lp = (unsigned)bcast((int)payload, leader);
     for (int base = lb; base < le; base += 64) fn(base + lane, base + lane < le, lp);
     mask &= mask - 1;
   }
@@ -126,20 +130,15 @@ __device__ __forceinline__ T wave_coop_walk(bool active, int b, int e, T identit
                                             int serial_limit) {
   const bool heavy = active && (e - b) > serial_limit;
   const int len = (active && !heavy) ? (e - b) : 0;
-  int maxlen = len;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int o = __shfl_xor(maxlen, off, 64);
-    maxlen = o > maxlen ? o : maxlen;
-  }
+  const int maxlen = wave_max(len);
   T acc = identity;
   for (int t = 0; t < maxlen; ++t) acc = red(acc, fn(b + t, t < len));
   unsigned long long mask = __ballot(heavy);
   const int lane = lane_id();
   while (mask) {
     const int leader = __ffsll((long long)mask) - 1;
-    const int lb = __shfl(b, leader, 64);
-    const int le = __shfl(e, leader, 64);
+    const int lb = bcast(b, leader);
+    const int le = bcast(e, leader);
     T part = identity;
     for (int base = lb; base < le; base += 64) part = red(part, fn(base + lane, base + lane < le));
 #pragma unroll
@@ -168,8 +167,8 @@ __device__ __forceinline__ bool wave_coop_any(bool active, int b, int e, P pred,
   const int lane = lane_id();
   while (mask) {
     const int leader = __ffsll((long long)mask) - 1;
-    const int lb = __shfl(b, leader, 64);
-    const int le = __shfl(e, leader, 64);
+    const int lb = bcast(b, leader);
+    const int le = bcast(e, leader);
     bool hit = false;
     for (int base = lb; base < le; base += 64) {
       const bool h = base + lane < le && pred(base + lane);
@@ -192,7 +191,7 @@ __device__ __forceinline__ void wave_push(bool push, int item, int* queue, int* 
   const int lane = lane_id();
   int base = 0;
   if (lane == __ffsll((long long)m) - 1) base = fadd_rlx(tail, __popcll(m));
-  base = __shfl(base, __ffsll((long long)m) - 1, 64);
+  base = bcast(base, __ffsll((long long)m) - 1);
   if (push) queue[base + __popcll(m & ((1ull << lane) - 1ull))] = item;
 }
 
@@ -211,7 +210,7 @@ __device__ __forceinline__ void wave_push_n(const bool (&push)[N], const int (&i
   const int lane = lane_id();
   int base = 0;
   if (lane == 0) base = fadd_rlx(tail, cnt);
-  base = __shfl(base, 0, 64);
+  base = bcast(base, 0);
   const unsigned long long lt = (1ull << lane) - 1ull;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
@@ -220,10 +219,6 @@ __device__ __forceinline__ void wave_push_n(const bool (&push)[N], const int (&i
   }
 }
 
-// Wave-wide max over the active lanes, by DPP row shifts (no LDS round
-// trips), from the device library.
-extern "C" __device__ int __ockl_wfred_max_i32(int);
-__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
 
 // Wave-wide OR of a predicate.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
@@ -258,8 +253,8 @@ __device__ __forceinline__ T wave_walk_n(bool active, int b, int e, T identity, 
   const int lane = lane_id();
   while (mask) {
     const int leader = __ffsll((long long)mask) - 1;
-    const int lb = __shfl(b, leader, 64);
-    const int le = __shfl(e, leader, 64);
+    const int lb = bcast(b, leader);
+    const int le = bcast(e, leader);
     T part = identity;
     for (int base = lb; base < le; base += 64 * N) {
       bool ok[N];
