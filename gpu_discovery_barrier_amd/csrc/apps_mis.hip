@@ -83,6 +83,8 @@ __global__ __launch_bounds__(WGS) void k_mis_plain(gdb_kernel_ctx_t* ctx, const 
   }
 }
 
+constexpr int kSlots = 4;  // edges a lane gathers per step in the tuned walks
+
 // Packed per-vertex state for one 8-byte gather per edge: x = status
 // (-1 undecided, -2 decided), y = node_value bits.
 template <int WGS>
@@ -94,6 +96,9 @@ __global__ __launch_bounds__(WGS) void k_mis_tuned(gdb_kernel_ctx_t* ctx, const 
   const int tid = p_get_global_id(ctx, &local_ctx);
   const int stride = p_get_global_size(ctx, &local_ctx);
   const int lane = gdb::lane_id();
+  // Unused edge slots gather this thread's spare vertex, so that a step's
+  // loads are issued branch-free, before any of them is used.
+  const int spare = n > 0 ? (int)((unsigned)tid % (unsigned)n) : 0;
   int* qin = q0;
   int* qout = q1;
   int rounds = 0, nbar = 0;
@@ -120,12 +125,25 @@ __global__ __launch_bounds__(WGS) void k_mis_tuned(gdb_kernel_ctx_t* ctx, const 
         b = row[v];
         e = row[v + 1];
       }
-      auto edge = [&](int k) -> float {
-        const int2 x = cv[col[k]];
-        return x.x == -1 ? __int_as_float(x.y) : kBigNum;
+      // kSlots edges per lane step: the column loads, then the 8-byte
+      // (status, value) gathers, then the compares.
+      auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> float {
+        int u[kSlots];
+        long long x[kSlots];
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) u[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));  // edge 0 exists here
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) x[j] = gdb::ldg((const long long*)(cv + (ok[j] ? u[j] : spare)));
+        float m = kBigNum;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+          const float vj = __int_as_float((int)(x[j] >> 32));
+          if (ok[j] && (int)(x[j] & 0xffffffffll) == -1 && vj < m) m = vj;
+        }
+        return m;
       };
       auto red = [](float a, float b2) { return a < b2 ? a : b2; };
-      const float mn = gdb::wave_coop_reduce(valid != 0, b, e, kBigNum, edge, red, kSerialLimit);
+      const float mn = gdb::wave_walk_n<kSlots>(valid != 0, b, e, kBigNum, step, red, kSerialLimit);
       if (valid && __int_as_float(cv[v].y) < mn) s[v] = 2;
     }
     if (tid == 0) gdb::st_rlx(&tails[(rounds + 1) % 3], 0);
@@ -147,9 +165,19 @@ __global__ __launch_bounds__(WGS) void k_mis_tuned(gdb_kernel_ctx_t* ctx, const 
           e = row[v + 1];
         }
       }
-      auto edge = [&](int k) -> int { return s[col[k]] == 2 ? 1 : 0; };
+      auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> int {
+        int u[kSlots], sj[kSlots];
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) u[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) sj[j] = gdb::ldg(s + (ok[j] ? u[j] : spare));
+        int h = 0;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) h |= (ok[j] && sj[j] == 2) ? 1 : 0;
+        return h;
+      };
       auto red = [](int a, int b2) { return a | b2; };
-      const int hit = gdb::wave_coop_reduce(valid && !joined, b, e, 0, edge, red, kSerialLimit);
+      const int hit = gdb::wave_walk_n<kSlots>(valid && !joined, b, e, 0, step, red, kSerialLimit);
       const bool retire = valid && (joined || hit);
       if (retire) cv[v].x = -2;
       const bool keep = valid && !retire;
