@@ -213,15 +213,24 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
                     in[j] = ok[j] && (unsigned)w[j] < (unsigned)n;
                     x[j] = ld_state(ds, in[j] ? w[j] : spare);
                   }
+                  bool any_claim = false;
 #pragma unroll
                   for (int j = 0; j < kChunk; ++j) {
                     claim[j] = in[j] && x[j].x == -1;
-                    old[j] = gdb::atomic_cas_g(&ds[claim[j] ? w[j] : spare].x, claim[j] ? -1 : INT_MIN,
-                                               claim[j] ? l + 1 : INT_MIN);
+                    any_claim |= claim[j];
                     if (pull_sigma && in[j] && x[j].x == l - 1) acc += __int_as_float(x[j].y);
                   }
+                  // Most targets are claimed already: only lanes with a claim
+                  // send compare-and-swaps (an atomic, even a failing one,
+                  // dirties its line).  The results are consumed inside.
+                  if (any_claim) {
 #pragma unroll
-                  for (int j = 0; j < kChunk; ++j) push[j] = claim[j] && old[j] == -1;
+                    for (int j = 0; j < kChunk; ++j)
+                      old[j] = gdb::atomic_cas_g(&ds[claim[j] ? w[j] : spare].x, claim[j] ? -1 : INT_MIN,
+                                                 claim[j] ? l + 1 : INT_MIN);
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j) push[j] = claim[j] && old[j] == -1;
+                  }
                 }
                 gdb::wave_push_n<kChunk>(push, w, q, tail);
                 return acc;
