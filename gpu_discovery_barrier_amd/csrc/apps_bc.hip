@@ -249,6 +249,7 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
           sg = gdb::wave_walk_n<kChunk>(
               valid != 0 && l > 0, bt, et, 0.f,
               [&](const bool (&ok)[kChunk], const int (&kk)[kChunk]) -> float {
+                if (!gdb::any_slot(ok)) return 0.f;  // no slot: no loads
                 int u[kChunk];
                 int2 x[kChunk];
 #pragma unroll
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
         const float acc = gdb::wave_walk_n<kChunk>(
             valid != 0, b, e, 0.f,
             [&](const bool (&ok)[kChunk], const int (&kk)[kChunk]) -> float {
+              if (!gdb::any_slot(ok)) return 0.f;  // no slot: no loads
               int v[kChunk];
               int2 x[kChunk];
 #pragma unroll

@@ -156,6 +156,7 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
       // kSlots edges per lane step: the column loads, then the three
       // gathers per neighbour, then the compares.
       auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> gdb::MaxFound {
+        if (!gdb::any_slot(ok)) return gdb::MaxFound{-1.f, 0};  // no slot: no loads
         int u[kSlots], cu[kSlots], mu[kSlots];
         float xu[kSlots];
 #pragma unroll

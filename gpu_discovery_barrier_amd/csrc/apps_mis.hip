@@ -128,6 +128,7 @@ __global__ __launch_bounds__(WGS) void k_mis_tuned(gdb_kernel_ctx_t* ctx, const 
       // kSlots edges per lane step: the column loads, then the 8-byte
       // (status, value) gathers, then the compares.
       auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> float {
+        if (!gdb::any_slot(ok)) return kBigNum;  // no slot: no loads
         int u[kSlots];
         long long x[kSlots];
 #pragma unroll
@@ -166,6 +167,7 @@ __global__ __launch_bounds__(WGS) void k_mis_tuned(gdb_kernel_ctx_t* ctx, const 
         }
       }
       auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> int {
+        if (!gdb::any_slot(ok)) return 0;  // no slot: no loads
         int u[kSlots], sj[kSlots];
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) u[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));

@@ -223,6 +223,16 @@ __device__ __forceinline__ void wave_push_n(const bool (&push)[N], const int (&i
 // Wave-wide OR of a predicate.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// Whether a lane has any edge slot in a walk step (lanes without one can
+// skip the step's loads).
+template <int N>
+__device__ __forceinline__ bool any_slot(const bool (&ok)[N]) {
+  bool a = false;
+#pragma unroll
+  for (int j = 0; j < N; ++j) a |= ok[j];
+  return a;
+}
+
 // Chunked cooperative walk over the edge ranges [b, e) of the lanes of one
 // wave: step(ok, kk) handles N edge slots of the calling lane at once (slot j
 // is edge kk[j] when ok[j]) and returns the lane's contribution; it is called
