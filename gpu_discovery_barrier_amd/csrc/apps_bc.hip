@@ -354,6 +354,277 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
   }
 }
 
+// ---- multi-source BC on a symmetric graph -----------------------------------
+// kMS sources share one sweep.  Per vertex one 32-byte record: the kMS
+// sources' levels (-1 unreached) in st[2v] and their sigma bits (forward),
+// then (1 + delta) / sigma bits (backward, once delta is final) in st[2v+1].
+// One gather of a neighbour's record serves every source, and the sweep
+// reads the column array once for kMS sources.  Step t's queue range holds
+// the vertices that sit on level t for at least one source; walking such a
+// vertex pulls sigma from level t-1 and claims level t+1 for exactly those
+// sources.  The backward pass walks the steps from the deepest down: a
+// vertex on level t of source s takes delta_s = sigma_s * sum of coef_s over
+// neighbours on level t+1 (none on its source's deepest level: delta 0).
+constexpr int kMS = 4;
+
+struct F4 {  // per-source partial sums
+  float v[kMS];
+};
+__device__ __forceinline__ F4 shfl_xor(F4 x, int o) {
+  F4 r;
+#pragma unroll
+  for (int s = 0; s < kMS; ++s) r.v[s] = __shfl_xor(x.v[s], o, 64);
+  return r;
+}
+__device__ __forceinline__ F4 f4_add(F4 a, F4 b) {
+#pragma unroll
+  for (int s = 0; s < kMS; ++s) a.v[s] += b.v[s];
+  return a;
+}
+__device__ __forceinline__ int lane4(const int4& x, int s) {
+  return s == 0 ? x.x : s == 1 ? x.y : s == 2 ? x.z : x.w;
+}
+
+template <int WGS>
+__global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
+                                                  const int* __restrict__ col, int4* st, float* bc,
+                                                  const int* sources, int nsrc, int* q, int* stamp,
+                                                  int* loff, int* tail, int n, int* out) {
+  DISCOVERY_PROTOCOL(ctx);
+  const int tid = p_get_global_id(ctx, &local_ctx);
+  const int stride = p_get_global_size(ctx, &local_ctx);
+  // Unused edge slots read this thread's spare vertex (branch-free slots); a
+  // no-op compare-and-swap there compares against INT_MIN, which no level is.
+  const int spare = (int)((unsigned)tid % (unsigned)n);
+  const int qcap = kMS * n;  // a vertex is queued at most once per source level
+  int nbar = 0, levels = 0;
+  for (int i = tid; i < n; i += stride) {
+    bc[i] = 0.f;
+    st[2 * (size_t)i] = make_int4(-1, -1, -1, -1);
+    st[2 * (size_t)i + 1] = make_int4(0, 0, 0, 0);
+    stamp[i] = -1;
+  }
+  int prev_reached = 0;
+  for (int s0 = 0; s0 < nsrc; s0 += kMS) {
+    const int ns = nsrc - s0 < kMS ? nsrc - s0 : kMS;
+    for (int i = tid; i < prev_reached; i += stride) {  // what the previous sweep touched
+      const int v = q[i];
+      if ((unsigned)v < (unsigned)n) {
+        st[2 * (size_t)v] = make_int4(-1, -1, -1, -1);
+        st[2 * (size_t)v + 1] = make_int4(0, 0, 0, 0);
+        stamp[v] = -1;
+      }
+    }
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    if (tid == 0) {
+      int cnt = 0;
+      for (int j = 0; j < ns; ++j) {
+        const int s = sources[s0 + j];
+        int* rec = (int*)&st[2 * (size_t)s];
+        rec[j] = 0;
+        rec[4 + j] = __float_as_int(1.f);
+        if (stamp[s] != 0) {
+          stamp[s] = 0;
+          q[cnt++] = s;
+        }
+      }
+      loff[0] = 0;
+      loff[1] = cnt;
+      gdb::st_rlx(tail, cnt);
+    }
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    const unsigned all = (1u << ns) - 1u;
+    // Forward.
+    int lb = 0, le = gdb::ld_rlx(tail), t = 0;
+    for (;;) {
+      GDB_FOR_UNIFORM(i, valid, tid, stride, le - lb) {
+        int v = -1, b = 0, e = 0;
+        unsigned mask = 0;
+        if (valid) {
+          v = q[lb + i];
+          if ((unsigned)v >= (unsigned)n) valid = 0;
+        }
+        if (valid) {
+          const int4 lv = st[2 * (size_t)v];
+#pragma unroll
+          for (int s = 0; s < kMS; ++s) mask |= (lane4(lv, s) == t ? 1u : 0u) << s;
+          mask &= all;
+          b = row[v];
+          e = row[v + 1];
+        }
+        // On a long row every lane works for the row's owner: `om` is the
+        // owner's source mask.
+        const F4 sg = gdb::wave_walk_np<kChunk>(
+            valid && mask != 0, b, e, (int)mask, F4{{0.f, 0.f, 0.f, 0.f}},
+            [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], int om) -> F4 {
+              int w[kChunk];
+              bool push[kChunk];
+              F4 acc{{0.f, 0.f, 0.f, 0.f}};
+              bool any = false;
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j) {
+                w[j] = -1;
+                push[j] = false;
+                any |= ok[j];
+              }
+              if (any) {  // lanes without a slot this step stay out
+                int4 lw[kChunk], sw[kChunk];
+                bool in[kChunk];
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                  in[j] = ok[j] && (unsigned)w[j] < (unsigned)n;
+                  const size_t r = 2 * (size_t)(in[j] ? w[j] : spare);
+                  lw[j] = st[r];
+                  sw[j] = st[r + 1];
+                }
+                unsigned claim[kChunk];
+                bool any_claim = false;
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                  claim[j] = 0;
+#pragma unroll
+                  for (int s = 0; s < kMS; ++s) {
+                    const int ls = lane4(lw[j], s);
+                    const bool mine = in[j] && ((om >> s) & 1);
+                    if (mine && ls == t - 1) acc.v[s] += __int_as_float(lane4(sw[j], s));
+                    claim[j] |= (mine && ls == -1 ? 1u : 0u) << s;
+                  }
+                  any_claim |= claim[j] != 0;
+                }
+                // Only lanes with an unreached target send claims (an
+                // atomic, even a failing one, dirties its line); results are
+                // consumed inside.
+                if (any_claim) {
+                  int old[kChunk][kMS];
+#pragma unroll
+                  for (int j = 0; j < kChunk; ++j)
+#pragma unroll
+                    for (int s = 0; s < kMS; ++s) {
+                      const bool c = (claim[j] >> s) & 1u;
+                      int* lp = (int*)&st[2 * (size_t)(c ? w[j] : spare)] + s;
+                      old[j][s] = gdb::atomic_cas_g(lp, c ? -1 : INT_MIN, c ? t + 1 : INT_MIN);
+                    }
+                  bool won[kChunk];
+                  bool any_won = false;
+#pragma unroll
+                  for (int j = 0; j < kChunk; ++j) {
+                    won[j] = false;
+#pragma unroll
+                    for (int s = 0; s < kMS; ++s) won[j] |= ((claim[j] >> s) & 1u) && old[j][s] == -1;
+                    any_won |= won[j];
+                  }
+                  // A claimed vertex joins step t+1's queue once.
+                  if (any_won) {
+                    int sp[kChunk];
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j)
+                      sp[j] = gdb::atomic_max_g(&stamp[won[j] ? w[j] : spare], won[j] ? t + 1 : INT_MIN);
+#pragma unroll
+                    for (int j = 0; j < kChunk; ++j) push[j] = won[j] && sp[j] < t + 1;
+                  }
+                }
+              }
+              gdb::wave_push_n<kChunk>(push, w, q, tail);
+              return acc;
+            },
+            [](F4 a, F4 c) { return f4_add(a, c); }, kSerialLimit);
+        if (valid && t > 0) {
+          int* rec = (int*)&st[2 * (size_t)v + 1];
+#pragma unroll
+          for (int s = 0; s < kMS; ++s)
+            if ((mask >> s) & 1u) rec[s] = __float_as_int(sg.v[s]);
+        }
+      }
+      if (!discovery_barrier(ctx, &local_ctx)) return;
+      ++nbar;
+      const int tn = gdb::ld_rlx(tail);
+      if (tn > qcap || tn < le) {  // stamps queue a vertex once per step
+        gdb::set_error(ctx, GDB_ERR_QUEUE_OVERFLOW);
+        return;
+      }
+      if (tn == le) break;  // step t claimed nothing: t is the deepest step
+      lb = le;
+      le = tn;
+      ++t;
+      if (tid == 0) loff[t + 1] = le;
+    }
+    const int reached = le;
+    levels += t + 1;
+    // Backward, deepest step first (loff is visible after the last barrier).
+    for (int tt = t; tt >= 0; --tt) {
+      const int qb = tt == t ? lb : loff[tt];
+      const int qe = tt == t ? le : loff[tt + 1];
+      GDB_FOR_UNIFORM(i, valid, tid, stride, qe - qb) {
+        int v = -1, b = 0, e = 0;
+        unsigned mask = 0;
+        int4 sv = make_int4(0, 0, 0, 0);
+        if (valid) {
+          v = q[qb + i];
+          if ((unsigned)v >= (unsigned)n) valid = 0;
+        }
+        if (valid) {
+          const int4 lv = st[2 * (size_t)v];
+          sv = st[2 * (size_t)v + 1];
+#pragma unroll
+          for (int s = 0; s < kMS; ++s) mask |= (lane4(lv, s) == tt ? 1u : 0u) << s;
+          mask &= all;
+          b = row[v];
+          e = row[v + 1];
+        }
+        const F4 acc = gdb::wave_walk_np<kChunk>(
+            valid && mask != 0, b, e, (int)mask, F4{{0.f, 0.f, 0.f, 0.f}},
+            [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], int om) -> F4 {
+              F4 a{{0.f, 0.f, 0.f, 0.f}};
+              if (!gdb::any_slot(ok)) return a;  // no slot: no loads
+              int w[kChunk];
+              int4 lw[kChunk], sw[kChunk];
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j) {
+                const bool in = ok[j] && (unsigned)w[j] < (unsigned)n;
+                const size_t r = 2 * (size_t)(in ? w[j] : spare);
+                lw[j] = st[r];
+                sw[j] = st[r + 1];
+                if (!in) lw[j] = make_int4(-1, -1, -1, -1);
+              }
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j)
+#pragma unroll
+                for (int s = 0; s < kMS; ++s)
+                  if (((om >> s) & 1) && lane4(lw[j], s) == tt + 1) a.v[s] += __int_as_float(lane4(sw[j], s));
+              return a;
+            },
+            [](F4 x, F4 c) { return f4_add(x, c); }, kSerialLimit);
+        if (valid && mask) {
+          int* rec = (int*)&st[2 * (size_t)v + 1];
+          float add = 0.f;
+#pragma unroll
+          for (int s = 0; s < kMS; ++s)
+            if ((mask >> s) & 1u) {
+              const float sw = __int_as_float(lane4(sv, s));
+              const float dw = sw * acc.v[s];
+              if (v != sources[s0 + s]) add += dw;
+              rec[s] = __float_as_int((1.f + dw) / sw);
+            }
+          bc[v] += add;
+        }
+      }
+      if (!discovery_barrier(ctx, &local_ctx)) return;
+      ++nbar;
+    }
+    prev_reached = reached;
+  }
+  if (tid == 0) {
+    out[0] = levels;
+    out[1] = nbar;
+  }
+}
+
 __global__ void k_bc_clean(int* d, float* rho, float* delta, int s, int n, int* cont) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {  // clean_1d_array
@@ -418,8 +689,17 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
   if (nsrc) GDB_HIP(hipMemcpy(hsrc.data(), sources, sizeof(int) * nsrc, hipMemcpyDeviceToHost));
   for (int s : hsrc)
     if (s < 0 || s >= n) return GDB_E_INVALID;
+  // The tuned persistent schedule sweeps kMS sources at once on a symmetric
+  // graph (the CSR passed as its own transpose); a directed graph takes the
+  // single-source kernel with separate in- and out-edge walks.
+  const bool multi = o.variant == GDB_VARIANT_TUNED && o.schedule != GDB_SCHED_RELAUNCH && row == row_t &&
+                     col == col_t;
+  const size_t qcap = (size_t)kMS * n;
   const size_t bytes = carve_bytes<int>(n) * 2 + carve_bytes<float>(n) * 2 + carve_bytes<int>(32) +
-                       carve_bytes<int2>(n);
+                       carve_bytes<int2>(n) +
+                       (multi ? carve_bytes<int4>((size_t)2 * n) + carve_bytes<int>(qcap) +
+                                    carve_bytes<int>(qcap + 2)
+                              : 0);
   int rc = ensure_workspace(ctx, bytes);
   if (rc) return rc;
   Carve cv(ctx->ws);
@@ -429,6 +709,9 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
   float* delta = cv.take<float>(n);
   int* flags = cv.take<int>(32);
   int2* ds = cv.take<int2>(n);
+  int4* mst = multi ? cv.take<int4>((size_t)2 * n) : nullptr;
+  int* mq = multi ? cv.take<int>(qcap) : nullptr;
+  int* mloff = multi ? cv.take<int>(qcap + 2) : nullptr;
   rc = fill_i32(ctx, flags, 0, 32);
   if (rc) return rc;
   rc = prepare_launch(ctx);
@@ -471,13 +754,18 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
     int* out = flags + 8;
     rc = gdb_dispatch_wgs(wgs, [&](auto W) -> int {
       constexpr int WG = decltype(W)::value;
-      const void* kern = tuned ? (const void*)k_bc_tuned<WG> : (const void*)k_bc_plain<WG>;
+      const void* kern = multi   ? (const void*)k_bc_multi<WG>
+                         : tuned ? (const void*)k_bc_tuned<WG>
+                                 : (const void*)k_bc_plain<WG>;
       int groups = 0;
       int r2 = launch_groups(ctx, kern, &o, n, &groups);
       if (r2) return r2;
       r2 = timer_start(ctx);
       if (r2) return r2;
-      if (tuned)
+      if (multi)  // d is the per-vertex queue stamp here
+        hipLaunchKernelGGL(k_bc_multi<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col, mst,
+                           bc, sources, nsrc, mq, d, mloff, flags + 4, n, out);
+      else if (tuned)
         hipLaunchKernelGGL(k_bc_tuned<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col,
                            row_t, col_t, ds, bc, sources, nsrc, q, flags + 4, n, out);
       else

@@ -242,9 +242,12 @@ __device__ __forceinline__ bool any_slot(const bool (&ok)[N]) {
 // by their own lane N edges per step; long ranges by the whole wave, N x 64
 // consecutive edges per step (slot j of lane i takes edge base + i + 64 j, so
 // each slot's loads are coalesced), and reduced into the owner lane.
+// As wave_walk_n, but step(ok, kk, owner) also gets the `payload` of the
+// range's owner lane (for a long row walked by the whole wave, the leader's
+// payload on every lane).
 template <int N, typename T, typename Step, typename R>
-__device__ __forceinline__ T wave_walk_n(bool active, int b, int e, T identity, Step step, R red,
-                                         int serial_limit) {
+__device__ __forceinline__ T wave_walk_np(bool active, int b, int e, int payload, T identity, Step step, R red,
+                                          int serial_limit) {
   const bool heavy = active && (e - b) > serial_limit;
   const int len = (active && !heavy) ? (e - b) : 0;
   const int maxlen = wave_max(len);
@@ -257,7 +260,7 @@ __device__ __forceinline__ T wave_walk_n(bool active, int b, int e, T identity, 
       ok[j] = c + j < len;
       kk[j] = b + c + j;
     }
-    acc = red(acc, step(ok, kk));
+    acc = red(acc, step(ok, kk, payload));
   }
   unsigned long long mask = __ballot(heavy);
   const int lane = lane_id();
@@ -265,6 +268,7 @@ __device__ __forceinline__ T wave_walk_n(bool active, int b, int e, T identity, 
     const int leader = __ffsll((long long)mask) - 1;
     const int lb = bcast(b, leader);
     const int le = bcast(e, leader);
+    const int lp = bcast(payload, leader);
     T part = identity;
     for (int base = lb; base < le; base += 64 * N) {
       bool ok[N];
@@ -274,7 +278,7 @@ __device__ __forceinline__ T wave_walk_n(bool active, int b, int e, T identity, 
         kk[j] = base + lane + 64 * j;
         ok[j] = kk[j] < le;
       }
-      part = red(part, step(ok, kk));
+      part = red(part, step(ok, kk, lp));
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) part = red(part, shfl_xor(part, off));
@@ -282,6 +286,14 @@ __device__ __forceinline__ T wave_walk_n(bool active, int b, int e, T identity, 
     mask &= mask - 1;
   }
   return acc;
+}
+
+template <int N, typename T, typename Step, typename R>
+__device__ __forceinline__ T wave_walk_n(bool active, int b, int e, T identity, Step step, R red,
+                                         int serial_limit) {
+  return wave_walk_np<N>(
+      active, b, e, 0, identity, [&](const bool (&ok)[N], const int (&kk)[N], int) { return step(ok, kk); }, red,
+      serial_limit);
 }
 
 // Uniform-trip grid-stride loop over [0, n): every thread of a group runs
