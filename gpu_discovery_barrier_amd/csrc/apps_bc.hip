@@ -355,9 +355,9 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
 }
 
 // ---- multi-source BC on a symmetric graph -----------------------------------
-// kMS sources share one sweep.  Per vertex one 32-byte record: the kMS
-// sources' levels (-1 unreached) in st[2v] and their sigma bits (forward),
-// then (1 + delta) / sigma bits (backward, once delta is final) in st[2v+1].
+// kMS sources share one sweep.  Per vertex one record of 2 x kMS ints: the
+// kMS sources' levels (-1 unreached), then their sigma bits (forward), then
+// (1 + delta) / sigma bits (backward, once delta is final), as kNR int4 each.
 // One gather of a neighbour's record serves every source, and the sweep
 // reads the column array once for kMS sources.  Step t's queue range holds
 // the vertices that sit on level t for at least one source; walking such a
@@ -365,24 +365,44 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
 // sources.  The backward pass walks the steps from the deepest down: a
 // vertex on level t of source s takes delta_s = sigma_s * sum of coef_s over
 // neighbours on level t+1 (none on its source's deepest level: delta 0).
-constexpr int kMS = 4;
+constexpr int kMS = 8;
+constexpr int kNR = kMS / 4;             // int4 per half record
+constexpr int kSlotsM = kMS > 4 ? 2 : 4;  // edge slots per lane step (registers)
 
-struct F4 {  // per-source partial sums
+struct FV {  // per-source partial sums
   float v[kMS];
 };
-__device__ __forceinline__ F4 shfl_xor(F4 x, int o) {
-  F4 r;
+__device__ __forceinline__ FV shfl_xor(FV x, int o) {
+  FV r;
 #pragma unroll
   for (int s = 0; s < kMS; ++s) r.v[s] = __shfl_xor(x.v[s], o, 64);
   return r;
 }
-__device__ __forceinline__ F4 f4_add(F4 a, F4 b) {
+__device__ __forceinline__ FV fv_add(FV a, const FV& b) {
 #pragma unroll
   for (int s = 0; s < kMS; ++s) a.v[s] += b.v[s];
   return a;
 }
-__device__ __forceinline__ int lane4(const int4& x, int s) {
-  return s == 0 ? x.x : s == 1 ? x.y : s == 2 ? x.z : x.w;
+__device__ __forceinline__ FV fv_zero() {
+  FV r;
+#pragma unroll
+  for (int s = 0; s < kMS; ++s) r.v[s] = 0.f;
+  return r;
+}
+struct Half {  // kMS ints of a record
+  int4 q[kNR];
+};
+__device__ __forceinline__ int at(const Half& h, int s) {
+  const int4& x = h.q[s >> 2];
+  const int k = s & 3;
+  return k == 0 ? x.x : k == 1 ? x.y : k == 2 ? x.z : x.w;
+}
+__device__ __forceinline__ void ld_half(const int4* st, size_t rec, int part, Half& h) {
+#pragma unroll
+  for (int r = 0; r < kNR; ++r) h.q[r] = st[rec * (2 * kNR) + part * kNR + r];
+}
+__device__ __forceinline__ int* rec_int(int4* st, size_t rec, int part, int s) {
+  return (int*)&st[rec * (2 * kNR) + part * kNR] + s;
 }
 
 template <int WGS>
@@ -398,22 +418,24 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
   const int spare = (int)((unsigned)tid % (unsigned)n);
   const int qcap = kMS * n;  // a vertex is queued at most once per source level
   int nbar = 0, levels = 0;
+  auto reset = [&](int v) {
+#pragma unroll
+    for (int r = 0; r < kNR; ++r) {
+      st[(size_t)v * (2 * kNR) + r] = make_int4(-1, -1, -1, -1);
+      st[(size_t)v * (2 * kNR) + kNR + r] = make_int4(0, 0, 0, 0);
+    }
+    stamp[v] = -1;
+  };
   for (int i = tid; i < n; i += stride) {
     bc[i] = 0.f;
-    st[2 * (size_t)i] = make_int4(-1, -1, -1, -1);
-    st[2 * (size_t)i + 1] = make_int4(0, 0, 0, 0);
-    stamp[i] = -1;
+    reset(i);
   }
   int prev_reached = 0;
   for (int s0 = 0; s0 < nsrc; s0 += kMS) {
     const int ns = nsrc - s0 < kMS ? nsrc - s0 : kMS;
     for (int i = tid; i < prev_reached; i += stride) {  // what the previous sweep touched
       const int v = q[i];
-      if ((unsigned)v < (unsigned)n) {
-        st[2 * (size_t)v] = make_int4(-1, -1, -1, -1);
-        st[2 * (size_t)v + 1] = make_int4(0, 0, 0, 0);
-        stamp[v] = -1;
-      }
+      if ((unsigned)v < (unsigned)n) reset(v);
     }
     if (!discovery_barrier(ctx, &local_ctx)) return;
     ++nbar;
@@ -421,9 +443,8 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
       int cnt = 0;
       for (int j = 0; j < ns; ++j) {
         const int s = sources[s0 + j];
-        int* rec = (int*)&st[2 * (size_t)s];
-        rec[j] = 0;
-        rec[4 + j] = __float_as_int(1.f);
+        *rec_int(st, s, 0, j) = 0;
+        *rec_int(st, s, 1, j) = __float_as_int(1.f);
         if (stamp[s] != 0) {
           stamp[s] = 0;
           q[cnt++] = s;
@@ -435,7 +456,7 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
     }
     if (!discovery_barrier(ctx, &local_ctx)) return;
     ++nbar;
-    const unsigned all = (1u << ns) - 1u;
+    const unsigned all = ns >= 32 ? ~0u : (1u << ns) - 1u;
     // Forward.
     int lb = 0, le = gdb::ld_rlx(tail), t = 0;
     for (;;) {
@@ -447,50 +468,51 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
           if ((unsigned)v >= (unsigned)n) valid = 0;
         }
         if (valid) {
-          const int4 lv = st[2 * (size_t)v];
+          Half lv;
+          ld_half(st, v, 0, lv);
 #pragma unroll
-          for (int s = 0; s < kMS; ++s) mask |= (lane4(lv, s) == t ? 1u : 0u) << s;
+          for (int s = 0; s < kMS; ++s) mask |= (at(lv, s) == t ? 1u : 0u) << s;
           mask &= all;
           b = row[v];
           e = row[v + 1];
         }
         // On a long row every lane works for the row's owner: `om` is the
         // owner's source mask.
-        const F4 sg = gdb::wave_walk_np<kChunk>(
-            valid && mask != 0, b, e, (int)mask, F4{{0.f, 0.f, 0.f, 0.f}},
-            [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], int om) -> F4 {
-              int w[kChunk];
-              bool push[kChunk];
-              F4 acc{{0.f, 0.f, 0.f, 0.f}};
+        const FV sg = gdb::wave_walk_np<kSlotsM>(
+            valid && mask != 0, b, e, (int)mask, fv_zero(),
+            [&](const bool (&ok)[kSlotsM], const int (&kk)[kSlotsM], int om) -> FV {
+              int w[kSlotsM];
+              bool push[kSlotsM];
+              FV acc = fv_zero();
               bool any = false;
 #pragma unroll
-              for (int j = 0; j < kChunk; ++j) {
+              for (int j = 0; j < kSlotsM; ++j) {
                 w[j] = -1;
                 push[j] = false;
                 any |= ok[j];
               }
               if (any) {  // lanes without a slot this step stay out
-                int4 lw[kChunk], sw[kChunk];
-                bool in[kChunk];
+                Half lw[kSlotsM], sw[kSlotsM];
+                bool in[kSlotsM];
 #pragma unroll
-                for (int j = 0; j < kChunk; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
+                for (int j = 0; j < kSlotsM; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
 #pragma unroll
-                for (int j = 0; j < kChunk; ++j) {
+                for (int j = 0; j < kSlotsM; ++j) {
                   in[j] = ok[j] && (unsigned)w[j] < (unsigned)n;
-                  const size_t r = 2 * (size_t)(in[j] ? w[j] : spare);
-                  lw[j] = st[r];
-                  sw[j] = st[r + 1];
+                  const size_t r = (size_t)(in[j] ? w[j] : spare);
+                  ld_half(st, r, 0, lw[j]);
+                  ld_half(st, r, 1, sw[j]);
                 }
-                unsigned claim[kChunk];
+                unsigned claim[kSlotsM];
                 bool any_claim = false;
 #pragma unroll
-                for (int j = 0; j < kChunk; ++j) {
+                for (int j = 0; j < kSlotsM; ++j) {
                   claim[j] = 0;
 #pragma unroll
                   for (int s = 0; s < kMS; ++s) {
-                    const int ls = lane4(lw[j], s);
+                    const int ls = at(lw[j], s);
                     const bool mine = in[j] && ((om >> s) & 1);
-                    if (mine && ls == t - 1) acc.v[s] += __int_as_float(lane4(sw[j], s));
+                    if (mine && ls == t - 1) acc.v[s] += __int_as_float(at(sw[j], s));
                     claim[j] |= (mine && ls == -1 ? 1u : 0u) << s;
                   }
                   any_claim |= claim[j] != 0;
@@ -499,19 +521,19 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
                 // atomic, even a failing one, dirties its line); results are
                 // consumed inside.
                 if (any_claim) {
-                  int old[kChunk][kMS];
+                  int old[kSlotsM][kMS];
 #pragma unroll
-                  for (int j = 0; j < kChunk; ++j)
+                  for (int j = 0; j < kSlotsM; ++j)
 #pragma unroll
                     for (int s = 0; s < kMS; ++s) {
                       const bool c = (claim[j] >> s) & 1u;
-                      int* lp = (int*)&st[2 * (size_t)(c ? w[j] : spare)] + s;
-                      old[j][s] = gdb::atomic_cas_g(lp, c ? -1 : INT_MIN, c ? t + 1 : INT_MIN);
+                      old[j][s] = gdb::atomic_cas_g(rec_int(st, c ? w[j] : spare, 0, s), c ? -1 : INT_MIN,
+                                                    c ? t + 1 : INT_MIN);
                     }
-                  bool won[kChunk];
+                  bool won[kSlotsM];
                   bool any_won = false;
 #pragma unroll
-                  for (int j = 0; j < kChunk; ++j) {
+                  for (int j = 0; j < kSlotsM; ++j) {
                     won[j] = false;
 #pragma unroll
                     for (int s = 0; s < kMS; ++s) won[j] |= ((claim[j] >> s) & 1u) && old[j][s] == -1;
@@ -519,24 +541,23 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
                   }
                   // A claimed vertex joins step t+1's queue once.
                   if (any_won) {
-                    int sp[kChunk];
+                    int sp[kSlotsM];
 #pragma unroll
-                    for (int j = 0; j < kChunk; ++j)
+                    for (int j = 0; j < kSlotsM; ++j)
                       sp[j] = gdb::atomic_max_g(&stamp[won[j] ? w[j] : spare], won[j] ? t + 1 : INT_MIN);
 #pragma unroll
-                    for (int j = 0; j < kChunk; ++j) push[j] = won[j] && sp[j] < t + 1;
+                    for (int j = 0; j < kSlotsM; ++j) push[j] = won[j] && sp[j] < t + 1;
                   }
                 }
               }
-              gdb::wave_push_n<kChunk>(push, w, q, tail);
+              gdb::wave_push_n<kSlotsM>(push, w, q, tail);
               return acc;
             },
-            [](F4 a, F4 c) { return f4_add(a, c); }, kSerialLimit);
+            [](FV a, FV c) { return fv_add(a, c); }, kSerialLimit);
         if (valid && t > 0) {
-          int* rec = (int*)&st[2 * (size_t)v + 1];
 #pragma unroll
           for (int s = 0; s < kMS; ++s)
-            if ((mask >> s) & 1u) rec[s] = __float_as_int(sg.v[s]);
+            if ((mask >> s) & 1u) *rec_int(st, v, 1, s) = __float_as_int(sg.v[s]);
         }
       }
       if (!discovery_barrier(ctx, &local_ctx)) return;
@@ -561,55 +582,59 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
       GDB_FOR_UNIFORM(i, valid, tid, stride, qe - qb) {
         int v = -1, b = 0, e = 0;
         unsigned mask = 0;
-        int4 sv = make_int4(0, 0, 0, 0);
+        Half sv;
+#pragma unroll
+        for (int r = 0; r < kNR; ++r) sv.q[r] = make_int4(0, 0, 0, 0);
         if (valid) {
           v = q[qb + i];
           if ((unsigned)v >= (unsigned)n) valid = 0;
         }
         if (valid) {
-          const int4 lv = st[2 * (size_t)v];
-          sv = st[2 * (size_t)v + 1];
+          Half lv;
+          ld_half(st, v, 0, lv);
+          ld_half(st, v, 1, sv);
 #pragma unroll
-          for (int s = 0; s < kMS; ++s) mask |= (lane4(lv, s) == tt ? 1u : 0u) << s;
+          for (int s = 0; s < kMS; ++s) mask |= (at(lv, s) == tt ? 1u : 0u) << s;
           mask &= all;
           b = row[v];
           e = row[v + 1];
         }
-        const F4 acc = gdb::wave_walk_np<kChunk>(
-            valid && mask != 0, b, e, (int)mask, F4{{0.f, 0.f, 0.f, 0.f}},
-            [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], int om) -> F4 {
-              F4 a{{0.f, 0.f, 0.f, 0.f}};
+        const FV acc = gdb::wave_walk_np<kSlotsM>(
+            valid && mask != 0, b, e, (int)mask, fv_zero(),
+            [&](const bool (&ok)[kSlotsM], const int (&kk)[kSlotsM], int om) -> FV {
+              FV a = fv_zero();
               if (!gdb::any_slot(ok)) return a;  // no slot: no loads
-              int w[kChunk];
-              int4 lw[kChunk], sw[kChunk];
+              int w[kSlotsM];
+              Half lw[kSlotsM], sw[kSlotsM];
 #pragma unroll
-              for (int j = 0; j < kChunk; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
+              for (int j = 0; j < kSlotsM; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
 #pragma unroll
-              for (int j = 0; j < kChunk; ++j) {
+              for (int j = 0; j < kSlotsM; ++j) {
                 const bool in = ok[j] && (unsigned)w[j] < (unsigned)n;
-                const size_t r = 2 * (size_t)(in ? w[j] : spare);
-                lw[j] = st[r];
-                sw[j] = st[r + 1];
-                if (!in) lw[j] = make_int4(-1, -1, -1, -1);
+                const size_t r = (size_t)(in ? w[j] : spare);
+                ld_half(st, r, 0, lw[j]);
+                ld_half(st, r, 1, sw[j]);
+                if (!in)
+#pragma unroll
+                  for (int rr = 0; rr < kNR; ++rr) lw[j].q[rr] = make_int4(-1, -1, -1, -1);
               }
 #pragma unroll
-              for (int j = 0; j < kChunk; ++j)
+              for (int j = 0; j < kSlotsM; ++j)
 #pragma unroll
                 for (int s = 0; s < kMS; ++s)
-                  if (((om >> s) & 1) && lane4(lw[j], s) == tt + 1) a.v[s] += __int_as_float(lane4(sw[j], s));
+                  if (((om >> s) & 1) && at(lw[j], s) == tt + 1) a.v[s] += __int_as_float(at(sw[j], s));
               return a;
             },
-            [](F4 x, F4 c) { return f4_add(x, c); }, kSerialLimit);
+            [](FV x, FV c) { return fv_add(x, c); }, kSerialLimit);
         if (valid && mask) {
-          int* rec = (int*)&st[2 * (size_t)v + 1];
           float add = 0.f;
 #pragma unroll
           for (int s = 0; s < kMS; ++s)
             if ((mask >> s) & 1u) {
-              const float sw = __int_as_float(lane4(sv, s));
+              const float sw = __int_as_float(at(sv, s));
               const float dw = sw * acc.v[s];
               if (v != sources[s0 + s]) add += dw;
-              rec[s] = __float_as_int((1.f + dw) / sw);
+              *rec_int(st, v, 1, s) = __float_as_int((1.f + dw) / sw);
             }
           bc[v] += add;
         }
@@ -697,7 +722,7 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
   const size_t qcap = (size_t)kMS * n;
   const size_t bytes = carve_bytes<int>(n) * 2 + carve_bytes<float>(n) * 2 + carve_bytes<int>(32) +
                        carve_bytes<int2>(n) +
-                       (multi ? carve_bytes<int4>((size_t)2 * n) + carve_bytes<int>(qcap) +
+                       (multi ? carve_bytes<int4>((size_t)2 * kNR * n) + carve_bytes<int>(qcap) +
                                     carve_bytes<int>(qcap + 2)
                               : 0);
   int rc = ensure_workspace(ctx, bytes);
@@ -709,7 +734,7 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
   float* delta = cv.take<float>(n);
   int* flags = cv.take<int>(32);
   int2* ds = cv.take<int2>(n);
-  int4* mst = multi ? cv.take<int4>((size_t)2 * n) : nullptr;
+  int4* mst = multi ? cv.take<int4>((size_t)2 * kNR * n) : nullptr;
   int* mq = multi ? cv.take<int>(qcap) : nullptr;
   int* mloff = multi ? cv.take<int>(qcap + 2) : nullptr;
   rc = fill_i32(ctx, flags, 0, 32);
