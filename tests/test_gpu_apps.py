@@ -118,6 +118,20 @@ def test_bc_rmat(ctx, sched):
     bc_close(got, want)
 
 
+def test_bc_sweeps_partial_batch_and_repeated_sources(ctx):
+    # Eight sources per sweep: 13 sources leave a partial last sweep, and a
+    # source listed twice (within one sweep and across two) is counted twice,
+    # as the single-source kernels and the oracle count it.
+    g = gdb.gen_rmat(12, 16, seed=21)
+    p = gdb.permutation(g.n, 21)
+    src = np.concatenate([p[:5], p[:1], p[5:11], p[1:2]]).astype(np.int32)  # repeats in and across sweeps
+    d = dev(g.row, g.col)
+    want = oracle.bc_brandes_tb(g.row, g.col, src)
+    for sched in SCHEDS[:3]:
+        got, _ = gdb.bc(ctx, d, d, src, gdb.Opts.make(**sched))
+        bc_close(got, want)
+
+
 def test_bc_tuned_is_deterministic(ctx):
     g = gdb.gen_rmat(12, 16, seed=8)
     src = gdb.permutation(g.n, 8)[:8]

@@ -92,6 +92,22 @@ def test_batch_local_hops_single_group(ctx, hops):
     check(gs, kinds, outs, vals, rounds, label_correcting=True)
 
 
+@pytest.mark.parametrize("hops", [1024, 4096])
+def test_batch_large_hop_budgets(ctx, hops):
+    # Budgets far above a grid's diameter: one group may carry the whole
+    # wavefront through its LDS queue; distances stay exact.
+    gs = [gdb.gen_grid(160, 160, 0.05, seed=3, wlo=1, whi=255), gdb.gen_grid(64, 64, 0.0, seed=4, wlo=1, whi=9)]
+    kinds, outs, vals, rounds, st = run_batch(ctx, gs, 8, 1024, wgs=256, groups=16, hops=hops)
+    check(gs, kinds, outs, vals, rounds, label_correcting=True)
+
+
+def test_batch_rejects_hop_budget_above_limit(ctx):
+    gs = small_graphs(1, seed=6)
+    with pytest.raises(gdb.GdbError) as e:
+        run_batch(ctx, gs, 1, 0, hops=4097)
+    assert e.value.code == gdb.GDB_E_INVALID
+
+
 def test_batch_local_hops_large_frontier_overflows_lds(ctx):
     # R-MAT frontiers exceed the 2048-entry LDS queue: the spill path runs.
     gs = [gdb.gen_rmat(15, 16, seed=11, wlo=1, whi=255)]
