@@ -726,10 +726,12 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
       // The team's share of the iteration; improved vertices go to LDS
       // buffer 1 (local step 1 reads it).
       process(in, nin, tid, stride, 1, &lcnt[1]);
-      // Local steps: this group alone relaxes what it just improved.
+      // Local steps: this group alone relaxes what it just improved.  Only
+      // LDS is exchanged between its waves here (the relaxations are
+      // returning atomics; queue stores are read after the team barrier,
+      // which drains them), so a workgroup barrier suffices.
       int s = 1;
       for (; s <= hops; ++s) {
-        gdb::drain_stores();
         __syncthreads();
         const int ln = min(lcnt[s % 3], kLocalQ);
         if (ln == 0) break;
