@@ -158,7 +158,10 @@ int gdb_mis(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const fl
  * unreachable = 99999999. */
 int gdb_sssp_pull(gdb_ctx* ctx, int n, int m, const int* row_t, const int* col_t, const int* w_t,
                   int source, int* dist, const gdb_opts* opts, gdb_stats* stats);
-/* Brandes betweenness over `nsrc` sources (FP32 sigma / delta / bc). */
+/* Brandes betweenness over `nsrc` sources (FP32 sigma / delta / bc).  With the
+ * tuned persistent schedule a symmetric graph (pass the CSR as its own
+ * transpose: row_t == row, col_t == col) is swept eight sources at a time;
+ * otherwise one source at a time with separate in- and out-edge walks. */
 int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* row_t,
            const int* col_t, const int* sources, int nsrc, float* bc, const gdb_opts* opts,
            gdb_stats* stats);
