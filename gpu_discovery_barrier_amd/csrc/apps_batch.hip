@@ -820,14 +820,19 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
 }
 
 // ---- colour on a team (one barrier per round, as k_color_tuned) ------------
-// scratch: maxd[2n] (floats)
+// scratch: pk[n] (int4: colour, value bits, max_d of the even and odd rounds)
+// A vertex's words travel together, so a neighbour costs one 16-byte load
+// (with three groups per CU sharing the memory pipeline, instructions per
+// edge bound this path).  A racing read of a neighbour's record sees its
+// colour as -1 or r-1, which give the same answer, and round r never reads
+// the max_d word it writes.  The colours are copied to `out` at the end.
 __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
   const int n = k.n;
   int* color = (int*)k.out;
-  float* maxd = (float*)k.scratch;
+  int4* pk = (int4*)k.scratch;
   const float* nv = k.nv;
   const int tid = T.tid(), stride = T.stride();
-  for (int v = tid; v < n; v += stride) color[v] = -1;
+  for (int v = tid; v < n; v += stride) pk[v] = make_int4(-1, __float_as_int(nv[v]), 0, 0);
   if (tid == 0) {
     gdb::st_rlx(T.word(TW_STOP0), 0);
     gdb::st_rlx(T.word(TW_STOP1), 0);
@@ -836,28 +841,34 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
   if (!team_barrier(T)) return -1;
   int r = 1;
   for (;;) {
-    const float* md_prev = maxd + (size_t)((r - 1) & 1) * n;
-    float* md_cur = maxd + (size_t)(r & 1) * n;
+    const int cur = r & 1;  // max_d word written this round (z even, w odd)
     const bool have_prev = r > 1;
     if (tid == 0) gdb::st_rlx(T.word(TW_STOP0 + (r + 1) % 3), 0);
     bool st = false;
     GDB_FOR_UNIFORM(v, valid, tid, stride, n) {
       bool active = false;
       int b = 0, e = 0;
-      if (valid && color[v] == -1) {
-        if (have_prev && nv[v] > md_prev[v]) {
-          color[v] = r - 1;
-        } else {
-          active = true;
-          b = k.row[v];
-          e = k.row[v + 1];
+      float nvv = 0.f;
+      if (valid) {
+        const int4 p = pk[v];
+        nvv = __int_as_float(p.y);
+        const float mp = __int_as_float(cur ? p.z : p.w);
+        if (p.x == -1) {
+          if (have_prev && nvv > mp) {
+            ((int*)&pk[v])[0] = r - 1;
+          } else {
+            active = true;
+            b = k.row[v];
+            e = k.row[v + 1];
+          }
         }
       }
       const bool deg1 = (e - b) == 1;
       auto edge = [&](int kk) -> gdb::MaxFound {
-        const int u = k.col[kk];
-        const bool unc = color[u] == -1 && !(have_prev && nv[u] > md_prev[u]);
-        return unc ? gdb::MaxFound{nv[u], 1} : gdb::MaxFound{-1.f, 0};
+        const int4 p = pk[k.col[kk]];
+        const float xu = __int_as_float(p.y);
+        const bool unc = p.x == -1 && !(have_prev && xu > __int_as_float(cur ? p.z : p.w));
+        return unc ? gdb::MaxFound{xu, 1} : gdb::MaxFound{-1.f, 0};
       };
       auto red = [](gdb::MaxFound a, gdb::MaxFound c) {
         return gdb::MaxFound{a.mx > c.mx ? a.mx : c.mx, a.found | c.found};
@@ -865,7 +876,7 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
       const gdb::MaxFound res =
           gdb::wave_coop_reduce(active && !deg1, b, e, gdb::MaxFound{-1.f, 0}, edge, red, kSerialLimit);
       if (active) {
-        md_cur[v] = deg1 ? -1.f : res.mx;
+        ((int*)&pk[v])[cur ? 3 : 2] = __float_as_int(deg1 ? -1.f : res.mx);
         if (!deg1 && res.found) st = true;
       }
     }
@@ -878,9 +889,12 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
     }
     ++r;
   }
-  const float* md = maxd + (size_t)(r & 1) * n;
-  for (int v = tid; v < n; v += stride)
-    if (color[v] == -1 && nv[v] > md[v]) color[v] = r;
+  const int cur = r & 1;
+  for (int v = tid; v < n; v += stride) {
+    const int4 p = pk[v];
+    const float md = __int_as_float(cur ? p.w : p.z);
+    color[v] = (p.x == -1 && __int_as_float(p.y) > md) ? r : p.x;
+  }
   return r;
 }
 
@@ -1013,7 +1027,7 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
         a.hops < 0 || a.hops > kMaxHops || (a.trow && !a.tcol) ||
         (cfg.mode != GDB_SEL_NONE && a.algo != GDB_TASK_COLOR && a.n > kNinMask))
       return GDB_E_INVALID;
-    const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : 2) * a.n;
+    const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : a.algo == GDB_TASK_COLOR ? 4 : 2) * a.n;
     dt[t] = DevTask{a.algo, a.n, a.m, a.source, a.delta, a.hops, a.row, a.col, a.w, a.node_value, a.out,
                     nullptr, a.trow, a.tcol, a.tw};
     scratch_total += (ints + 63) & ~(size_t)63;
@@ -1046,7 +1060,8 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
   int* scratch = cv.take<int>(scratch_total);
   size_t off = 0;
   for (int t = 0; t < ntasks; ++t) {
-    const size_t ints = (size_t)(dt[t].algo == GDB_TASK_SSSP || dt[t].hops > 0 ? 9 : 2) * dt[t].n;
+    const size_t ints =
+        (size_t)(dt[t].algo == GDB_TASK_SSSP || dt[t].hops > 0 ? 9 : dt[t].algo == GDB_TASK_COLOR ? 4 : 2) * dt[t].n;
     dt[t].scratch = scratch + off;
     off += (ints + 63) & ~(size_t)63;
   }
