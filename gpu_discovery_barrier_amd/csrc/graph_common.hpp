@@ -120,35 +120,6 @@ __device__ __forceinline__ void wave_coop_foreach(bool active, int b, int e, uns
   }
 }
 
-// Thread / wave cooperative walk over the edge ranges [b, e) of the lanes of
-// one wave that also reduces: fn(k, valid) returns an edge's contribution
-// (the identity when !valid) and may use wave collectives, so every step is
-// wave-uniform.  Short ranges are walked by their own lane, long ranges by
-// all 64 lanes (64 consecutive edges per step) and reduced into the owner.
-template <typename T, typename F, typename R>
-__device__ __forceinline__ T wave_coop_walk(bool active, int b, int e, T identity, F fn, R red,
-                                            int serial_limit) {
-  const bool heavy = active && (e - b) > serial_limit;
-  const int len = (active && !heavy) ? (e - b) : 0;
-  const int maxlen = wave_max(len);
-  T acc = identity;
-  for (int t = 0; t < maxlen; ++t) acc = red(acc, fn(b + t, t < len));
-  unsigned long long mask = __ballot(heavy);
-  const int lane = lane_id();
-  while (mask) {
-    const int leader = __ffsll((long long)mask) - 1;
-    const int lb = bcast(b, leader);
-    const int le = bcast(e, leader);
-    T part = identity;
-    for (int base = lb; base < le; base += 64) part = red(part, fn(base + lane, base + lane < le));
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) part = red(part, shfl_xor(part, off));
-    if (lane == leader) acc = part;
-    mask &= mask - 1;
-  }
-  return acc;
-}
-
 // Wave cooperative "some edge k in [b, e) has pred(k)" with early exit:
 // short ranges are scanned by their own lane up to the first hit, long ones
 // by the whole wave, 64 consecutive edges per step, up to the first step
