@@ -469,6 +469,12 @@ def main() -> int:
     us_bar = (tk_ms - t0_ms) * 1e3 / K
     us_relaunch = ctx.relaunch_bench(wgs, P, K, 0) * 1e3 / K
     us_relaunch_sync = ctx.relaunch_bench(wgs, P, 500, 1) * 1e3 / 500
+    # The same at 32 groups, the smallest team shape (the barrier's cost grows
+    # with the groups it joins).
+    t0_32, _ = ctx.barrier_bench(wgs, 32, 0)
+    tk_32, pb32 = ctx.barrier_bench(wgs, 32, K)
+    us_bar32 = (tk_32 - t0_32) * 1e3 / K
+    us_relaunch32 = ctx.relaunch_bench(wgs, 32, K, 0) * 1e3 / K
 
     # Roofline of the dominant kernel (the batched kernel): algorithmic bytes
     # = BFS 8 B per processed edge + 24 B per processed vertex, SSSP 16 B per
@@ -512,7 +518,9 @@ def main() -> int:
         "verify_errors": int(nerr),
         "barrier": {"us_per_barrier": round(us_bar, 3), "us_per_relaunch": round(us_relaunch, 3),
                     "us_per_relaunch_with_readback": round(us_relaunch_sync, 3),
-                    "ratio": round(us_bar / us_relaunch, 3), "P": pb, "wgs": wgs},
+                    "ratio": round(us_bar / us_relaunch, 3), "P": pb, "wgs": wgs,
+                    "at_32_groups": {"us_per_barrier": round(us_bar32, 3), "us_per_relaunch": round(us_relaunch32, 3),
+                                     "ratio": round(us_bar32 / us_relaunch32, 3), "P": pb32}},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
