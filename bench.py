@@ -30,7 +30,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # wgs, groups, near/far band, local hops, selector mode (0 none, 4 MLP)
-TUNED_C2 = (256, 256, 4096, 1024, 0)
+TUNED_C2 = (128, 256, 4096, 256, 0)
 GRID_SIDE = 2048
 
 
@@ -408,12 +408,12 @@ def main() -> int:
         wgs, groups, delta, hops, mode = (int(x) for x in args.shape.split(","))
     elif not args.tune:
         # The autotuner's pick on this workload (python bench.py --tune):
-        # 120 shapes; the best few within 2% of each other.
+        # 180 shapes; the best few within 2-4% of each other.
         wgs, groups, delta, hops, mode = TUNED_C2
     else:
         trials = []
-        for w in (256, 512):
-            for gr in (32, 64, 128, 256):
+        for w in (128, 256, 512):
+            for gr in (64, 128, 256, 512):
                 gr = min(gr, gdb.max_resident_groups(w, 0))
                 for dl in (2048, 4096, 8192):
                     for hp in (0, 64, 256, 1024):
