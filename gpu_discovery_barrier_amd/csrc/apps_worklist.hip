@@ -33,6 +33,7 @@ namespace {
 
 constexpr unsigned kInf = 1000000000u;  // bfs_common.cl:35
 constexpr int kSerialLimit = 16;
+constexpr int kChunk = 4;  // edge slots a lane handles per step in the tuned kernel
 
 // Relaxes edge k out of v (distance dv).  Returns whether dst must be pushed.
 template <bool WEIGHTED>
@@ -127,6 +128,7 @@ __global__ __launch_bounds__(WGS) void k_wl_tuned(gdb_kernel_ctx_t* ctx, const i
   DISCOVERY_PROTOCOL(ctx);
   const int tid = p_get_global_id(ctx, &local_ctx);
   const int threads = p_get_global_size(ctx, &local_ctx);
+  const int spare = n > 0 ? (int)((unsigned)tid % (unsigned)n) : 0;
   int iteration = 1, nbar = 0;
   long long items = 0, edges = 0;
   int nin = gdb::ld_rlx(&tails[1]);
@@ -153,15 +155,79 @@ __global__ __launch_bounds__(WGS) void k_wl_tuned(gdb_kernel_ctx_t* ctx, const i
         if (WEIGHTED) dv = dist[v];
         edges += e - b;
       }
-      gdb::wave_coop_foreach(
-          valid != 0, b, e, dv,
-          [&](int k, bool ok, unsigned src_dist) {
-            int dst = 0;
-            const bool push =
-                ok && relax_edge<WEIGHTED>(col, w, dist, stamp, k, src_dist, iteration, n, &dst);
-            gdb::wave_push(push, dst, outq, out_tail);
+      // kChunk edges per lane step (long rows: the whole wave, carrying the
+      // row owner's distance): the column and weight loads, then the
+      // targets' distances, then atomics only where a candidate improves
+      // (lanes with none send no atomic), then the once-per-iteration stamps
+      // of the improved targets, then one wave push.  Unused slots read edge
+      // 0 and this thread's spare vertex, so that the slots stay branch-free.
+      gdb::wave_walk_np<kChunk>(
+          valid != 0, b, e, (int)dv, 0,
+          [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], int pd) -> int {
+            const unsigned src = (unsigned)pd;
+            int dst[kChunk];
+            bool push[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+              dst[j] = -1;
+              push[j] = false;
+            }
+            if (gdb::any_slot(ok)) {
+              unsigned wt[kChunk], cur[kChunk];
+              bool act[kChunk], need[kChunk];
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j) {
+                const int e0 = ok[j] ? kk[j] : 0;  // edge 0 exists here
+                dst[j] = gdb::ldg(col + e0);
+                wt[j] = WEIGHTED ? (unsigned)gdb::ldg(w + e0) : 1u;
+              }
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j) {
+                act[j] = ok[j] && (unsigned)dst[j] < (unsigned)n && wt[j] < kInf;
+                if (!ok[j]) dst[j] = -1;
+                cur[j] = gdb::ldg(dist + (act[j] ? dst[j] : spare));
+              }
+              // A stale distance is never below the current one, so it can
+              // only let a needless atomic through.
+              bool any_need = false;
+#pragma unroll
+              for (int j = 0; j < kChunk; ++j) {
+                need[j] = act[j] && (WEIGHTED ? src + wt[j] < cur[j] : cur[j] == kInf);
+                any_need |= need[j];
+              }
+              if (any_need) {
+                unsigned old[kChunk];
+                bool imp[kChunk], any_imp = false;
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                  unsigned* dp = &dist[need[j] ? dst[j] : spare];
+                  old[j] = WEIGHTED ? gdb::atomic_min_g(dp, need[j] ? src + wt[j] : ~0u)
+                                    : gdb::atomic_cas_g(dp, need[j] ? kInf : ~0u, need[j] ? (unsigned)iteration : ~0u);
+                }
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) {
+                  imp[j] = need[j] && (WEIGHTED ? src + wt[j] < old[j] : old[j] == kInf);
+                  any_imp |= imp[j];
+                }
+                if (!WEIGHTED) {
+#pragma unroll
+                  for (int j = 0; j < kChunk; ++j) push[j] = imp[j];
+                } else if (any_imp) {
+                  // Stamps only grow (iteration numbers): the first to raise
+                  // one to this iteration queues the target.
+                  int sp[kChunk];
+#pragma unroll
+                  for (int j = 0; j < kChunk; ++j)
+                    sp[j] = gdb::atomic_max_g(&stamp[imp[j] ? dst[j] : spare], imp[j] ? iteration : INT_MIN);
+#pragma unroll
+                  for (int j = 0; j < kChunk; ++j) push[j] = imp[j] && sp[j] < iteration;
+                }
+              }
+            }
+            gdb::wave_push_n<kChunk>(push, dst, outq, out_tail);
+            return 0;
           },
-          kSerialLimit);
+          [](int a, int c) { return a | c; }, kSerialLimit);
     }
     if (!discovery_barrier(ctx, &local_ctx)) return;
     ++nbar;
