@@ -181,14 +181,17 @@ def bench_c3(args) -> int:
     if args.engine_only:
         # Tuned kernels only, at the autotuner's shapes (profiler runs).
         m_ms = mis(gdb.VARIANT_TUNED, 256, 2048)[0]
-        b_ms = bc(gdb.VARIANT_TUNED, 256, 1024)[0]
+        b_ms = bc(gdb.VARIANT_TUNED, 256, 2048)[0]
         print(json.dumps({"mis_ms": round(m_ms, 3), "bc_ms": round(b_ms, 3)}), flush=True)
         ctx.close()
         return 0
     shapes = [(w, gr) for w in (256, 512, 1024) for gr in (256, 512, 1024, 2048)
               if gr <= gdb.max_resident_groups(w, 0)]
     mt = sorted((mis(gdb.VARIANT_TUNED, w, gr)[0], w, gr) for w, gr in shapes)[0]
-    bt = sorted((bc(gdb.VARIANT_TUNED, w, gr)[0], w, gr) for w, gr in shapes[::2])[0]
+    # BC runs take ~0.6 s each: a few shapes around the usual pick.
+    bshapes = [(w, gr) for w, gr in ((256, 1024), (256, 2048), (512, 512), (512, 1024))
+               if gr <= gdb.max_resident_groups(w, 0)]
+    bt = sorted((bc(gdb.VARIANT_TUNED, w, gr)[0], w, gr) for w, gr in bshapes)[0]
     log(f"autotune: mis wgs={mt[1]} groups={mt[2]}, bc wgs={bt[1]} groups={bt[2]}")
     m_ms, wall_m, (_, s_t, st_m) = timed(lambda: mis(gdb.VARIANT_TUNED, mt[1], mt[2]), args.warmup, args.steps)
     b_ms, wall_b, (_, b_t, st_b) = timed(lambda: bc(gdb.VARIANT_TUNED, bt[1], bt[2]), 1, max(1, args.steps // 2))
