@@ -29,6 +29,7 @@
 //                      loop (bc.cpp:322-419).
 #include "graph_common.hpp"
 #include "runtime.hpp"
+#include <climits>
 
 namespace {
 
@@ -718,7 +719,7 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
   // graph (the CSR passed as its own transpose); a directed graph takes the
   // single-source kernel with separate in- and out-edge walks.
   const bool multi = o.variant == GDB_VARIANT_TUNED && o.schedule != GDB_SCHED_RELAUNCH && row == row_t &&
-                     col == col_t;
+                     col == col_t && (long long)kMS * n + 2 <= INT_MAX;  // queue positions are ints
   const size_t qcap = (size_t)kMS * n;
   const size_t bytes = carve_bytes<int>(n) * 2 + carve_bytes<float>(n) * 2 + carve_bytes<int>(32) +
                        carve_bytes<int2>(n) +
