@@ -410,12 +410,12 @@ template <int WGS>
 __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
                                                   const int* __restrict__ col, int4* st, float* bc,
                                                   const int* sources, int nsrc, int* q, int* stamp,
-                                                  int* loff, int* tail, int n, int* out) {
+                                                  int* reach, int* loff, int* tail, int n, int* out) {
   DISCOVERY_PROTOCOL(ctx);
   const int tid = p_get_global_id(ctx, &local_ctx);
   const int stride = p_get_global_size(ctx, &local_ctx);
-  // Unused edge slots read this thread's spare vertex (branch-free slots); a
-  // no-op compare-and-swap there compares against INT_MIN, which no level is.
+  // Unused edge slots read this thread's spare vertex (branch-free slots) and
+  // send it no-op atomics there (an or of 0, a max with INT_MIN).
   const int spare = (int)((unsigned)tid % (unsigned)n);
   const int qcap = kMS * n;  // a vertex is queued at most once per source level
   int nbar = 0, levels = 0;
@@ -426,6 +426,7 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
       st[(size_t)v * (2 * kNR) + kNR + r] = make_int4(0, 0, 0, 0);
     }
     stamp[v] = -1;
+    reach[v] = 0;
   };
   for (int i = tid; i < n; i += stride) {
     bc[i] = 0.f;
@@ -446,6 +447,7 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
         const int s = sources[s0 + j];
         *rec_int(st, s, 0, j) = 0;
         *rec_int(st, s, 1, j) = __float_as_int(1.f);
+        reach[s] |= 1 << j;
         if (stamp[s] != 0) {
           stamp[s] = 0;
           q[cnt++] = s;
@@ -518,27 +520,26 @@ __global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const i
                   }
                   any_claim |= claim[j] != 0;
                 }
-                // Only lanes with an unreached target send claims (an
-                // atomic, even a failing one, dirties its line); results are
-                // consumed inside.
+                // Only lanes with an unreached target send claims.  One
+                // fetch-or on the target's reached mask claims all of the
+                // slot's sources at once (winners are the bits it sets), and
+                // each winner writes its level with a plain store: a reader
+                // that still sees -1 there loses at the mask.
                 if (any_claim) {
-                  int old[kSlotsM][kMS];
+                  unsigned oldm[kSlotsM];
 #pragma unroll
                   for (int j = 0; j < kSlotsM; ++j)
-#pragma unroll
-                    for (int s = 0; s < kMS; ++s) {
-                      const bool c = (claim[j] >> s) & 1u;
-                      old[j][s] = gdb::atomic_cas_g(rec_int(st, c ? w[j] : spare, 0, s), c ? -1 : INT_MIN,
-                                                    c ? t + 1 : INT_MIN);
-                    }
+                    oldm[j] = (unsigned)gdb::atomic_or_g(&reach[claim[j] ? w[j] : spare], (int)claim[j]);
                   bool won[kSlotsM];
                   bool any_won = false;
 #pragma unroll
                   for (int j = 0; j < kSlotsM; ++j) {
-                    won[j] = false;
-#pragma unroll
-                    for (int s = 0; s < kMS; ++s) won[j] |= ((claim[j] >> s) & 1u) && old[j][s] == -1;
+                    const unsigned got = claim[j] & ~oldm[j];
+                    won[j] = got != 0u;
                     any_won |= won[j];
+#pragma unroll
+                    for (int s = 0; s < kMS; ++s)
+                      if ((got >> s) & 1u) *rec_int(st, w[j], 0, s) = t + 1;
                   }
                   // A claimed vertex joins step t+1's queue once.
                   if (any_won) {
@@ -788,9 +789,9 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
       if (r2) return r2;
       r2 = timer_start(ctx);
       if (r2) return r2;
-      if (multi)  // d is the per-vertex queue stamp here
+      if (multi)  // d holds the per-vertex queue stamps and q the reached masks here
         hipLaunchKernelGGL(k_bc_multi<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col, mst,
-                           bc, sources, nsrc, mq, d, mloff, flags + 4, n, out);
+                           bc, sources, nsrc, mq, d, q, mloff, flags + 4, n, out);
       else if (tuned)
         hipLaunchKernelGGL(k_bc_tuned<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col,
                            row_t, col_t, ds, bc, sources, nsrc, q, flags + 4, n, out);

@@ -37,6 +37,10 @@ __device__ __forceinline__ unsigned long long atomic_min_g(unsigned long long* p
   return __hip_atomic_fetch_min((__attribute__((address_space(1))) unsigned long long*)p, v, __ATOMIC_RELAXED,
                                 __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ int atomic_or_g(int* p, int v) {
+  return __hip_atomic_fetch_or((__attribute__((address_space(1))) int*)p, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ int atomic_max_g(int* p, int v) {
   return __hip_atomic_fetch_max((__attribute__((address_space(1))) int*)p, v, __ATOMIC_RELAXED,
                                 __HIP_MEMORY_SCOPE_AGENT);
