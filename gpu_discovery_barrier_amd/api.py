@@ -554,8 +554,11 @@ def make_task(algo: int, g: "DeviceCSR", out: DeviceBuffer, source: int = 0,
               node_value: DeviceBuffer | None = None, delta: int = 0, hops: int = 0,
               transpose: "DeviceCSR | None" = None) -> Task:
     """`transpose` is the in-edge CSR (the graph itself when it is symmetric);
-    it enables the pull arms of the schedule selector."""
+    it enables the pull arms of the schedule selector.  The task holds
+    references to the device objects it points into, so they stay allocated
+    while the task is alive."""
     t = Task()
+    t._keep = (g, out, node_value, transpose)
     t.algo, t.n, t.m = algo, g.n, g.m
     t.row, t.col = g.row.ptr, g.col.ptr
     t.w = g.w.ptr if g.w is not None else None

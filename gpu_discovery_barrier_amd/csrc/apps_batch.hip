@@ -820,19 +820,30 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
 }
 
 // ---- colour on a team (one barrier per round, as k_color_tuned) ------------
-// scratch: pk[n] (int4: colour, value bits, max_d of the even and odd rounds)
-// A vertex's words travel together, so a neighbour costs one 16-byte load
-// (with three groups per CU sharing the memory pipeline, instructions per
-// edge bound this path).  A racing read of a neighbour's record sees its
-// colour as -1 or r-1, which give the same answer, and round r never reads
-// the max_d word it writes.  The colours are copied to `out` at the end.
+// scratch: pk[n] (int4: colour, value bits, max_d of the even and odd rounds),
+// then blk[n] (the neighbour that set max_d, -1 none).
+// A vertex's words travel together, so a neighbour costs one 16-byte load.
+// A racing read of a neighbour's record sees its colour as -1 or r-1, which
+// give the same answer, and round r never reads the max_d word it writes.
+// The colours are copied to `out` at the end.
+//
+// Blockers: a vertex's max_d is the largest value among its uncoloured
+// neighbours, and neighbours only ever leave that set.  While the neighbour
+// that held the maximum (the blocker) is still uncoloured, the maximum is
+// unchanged, so the vertex checks that one record instead of its row; it
+// walks its row again only once the blocker is coloured.  Colours, rounds
+// and max_d values equal those of the full walk every round.
 __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
   const int n = k.n;
   int* color = (int*)k.out;
   int4* pk = (int4*)k.scratch;
+  int* blk = (int*)(pk + n);
   const float* nv = k.nv;
   const int tid = T.tid(), stride = T.stride();
-  for (int v = tid; v < n; v += stride) pk[v] = make_int4(-1, __float_as_int(nv[v]), 0, 0);
+  for (int v = tid; v < n; v += stride) {
+    pk[v] = make_int4(-1, __float_as_int(nv[v]), 0, 0);
+    blk[v] = -1;
+  }
   if (tid == 0) {
     gdb::st_rlx(T.word(TW_STOP0), 0);
     gdb::st_rlx(T.word(TW_STOP1), 0);
@@ -844,14 +855,20 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
     const int cur = r & 1;  // max_d word written this round (z even, w odd)
     const bool have_prev = r > 1;
     if (tid == 0) gdb::st_rlx(T.word(TW_STOP0 + (r + 1) % 3), 0);
+    // Uncoloured as seen in round r: not coloured before, and not colouring
+    // now (value above the max_d of the previous round).
+    auto uncoloured = [&](const int4 p) {
+      return p.x == -1 && !(have_prev && __int_as_float(p.y) > __int_as_float(cur ? p.z : p.w));
+    };
     bool st = false;
     GDB_FOR_UNIFORM(v, valid, tid, stride, n) {
-      bool active = false;
-      int b = 0, e = 0;
-      float nvv = 0.f;
+      bool active = false, blocked = false;
+      int b = 0, e = 0, bk = -1;
+      int4 p = make_int4(0, 0, 0, 0);
       if (valid) {
-        const int4 p = pk[v];
-        nvv = __int_as_float(p.y);
+        p = pk[v];
+        bk = blk[v];
+        const float nvv = __int_as_float(p.y);
         const float mp = __int_as_float(cur ? p.z : p.w);
         if (p.x == -1) {
           if (have_prev && nvv > mp) {
@@ -860,24 +877,34 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
             active = true;
             b = k.row[v];
             e = k.row[v + 1];
+            if (bk >= 0) blocked = uncoloured(pk[bk]);
           }
         }
       }
       const bool deg1 = (e - b) == 1;
-      auto edge = [&](int kk) -> gdb::MaxFound {
-        const int4 p = pk[k.col[kk]];
-        const float xu = __int_as_float(p.y);
-        const bool unc = p.x == -1 && !(have_prev && xu > __int_as_float(cur ? p.z : p.w));
-        return unc ? gdb::MaxFound{xu, 1} : gdb::MaxFound{-1.f, 0};
+      auto edge = [&](int kk) -> gdb::MaxAt {
+        const int u = k.col[kk];
+        const int4 q = pk[u];
+        return uncoloured(q) ? gdb::MaxAt{__int_as_float(q.y), u} : gdb::MaxAt{-1.f, -1};
       };
-      auto red = [](gdb::MaxFound a, gdb::MaxFound c) {
-        return gdb::MaxFound{a.mx > c.mx ? a.mx : c.mx, a.found | c.found};
-      };
-      const gdb::MaxFound res =
-          gdb::wave_coop_reduce(active && !deg1, b, e, gdb::MaxFound{-1.f, 0}, edge, red, kSerialLimit);
+      auto red = [](gdb::MaxAt a, gdb::MaxAt c) { return (c.at >= 0 && (a.at < 0 || c.mx > a.mx)) ? c : a; };
+      const gdb::MaxAt res = gdb::wave_coop_reduce(active && !deg1 && !blocked, b, e, gdb::MaxAt{-1.f, -1}, edge,
+                                                   red, kSerialLimit);
       if (active) {
-        ((int*)&pk[v])[cur ? 3 : 2] = __float_as_int(deg1 ? -1.f : res.mx);
-        if (!deg1 && res.found) st = true;
+        if (blocked) {
+          // The maximum of the previous round stands; copy it to this
+          // round's word when they differ.
+          const int prev = cur ? p.z : p.w;
+          if ((cur ? p.w : p.z) != prev) ((int*)&pk[v])[cur ? 3 : 2] = prev;
+          st = true;
+        } else {
+          const float md = deg1 ? -1.f : (res.mx > -1.f ? res.mx : -1.f);
+          ((int*)&pk[v])[cur ? 3 : 2] = __float_as_int(md);
+          if (!deg1) {
+            if (res.at != bk) blk[v] = res.at;
+            if (res.at >= 0) st = true;
+          }
+        }
       }
     }
     if (gdb::wave_any(st) && gdb::lane_id() == 0) gdb::st_rlx(T.word(TW_STOP0 + r % 3), 1);
@@ -1027,7 +1054,7 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
         a.hops < 0 || a.hops > kMaxHops || (a.trow && !a.tcol) ||
         (cfg.mode != GDB_SEL_NONE && a.algo != GDB_TASK_COLOR && a.n > kNinMask))
       return GDB_E_INVALID;
-    const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : a.algo == GDB_TASK_COLOR ? 4 : 2) * a.n;
+    const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : a.algo == GDB_TASK_COLOR ? 5 : 2) * a.n;
     dt[t] = DevTask{a.algo, a.n, a.m, a.source, a.delta, a.hops, a.row, a.col, a.w, a.node_value, a.out,
                     nullptr, a.trow, a.tcol, a.tw};
     scratch_total += (ints + 63) & ~(size_t)63;
@@ -1061,7 +1088,7 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
   size_t off = 0;
   for (int t = 0; t < ntasks; ++t) {
     const size_t ints =
-        (size_t)(dt[t].algo == GDB_TASK_SSSP || dt[t].hops > 0 ? 9 : dt[t].algo == GDB_TASK_COLOR ? 4 : 2) * dt[t].n;
+        (size_t)(dt[t].algo == GDB_TASK_SSSP || dt[t].hops > 0 ? 9 : dt[t].algo == GDB_TASK_COLOR ? 5 : 2) * dt[t].n;
     dt[t].scratch = scratch + off;
     off += (ints + 63) & ~(size_t)63;
   }

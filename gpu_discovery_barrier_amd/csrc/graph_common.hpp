@@ -63,6 +63,13 @@ struct MaxFound {  // running max plus "at least one candidate seen"
 __device__ __forceinline__ MaxFound shfl_xor(MaxFound x, int o) {
   return MaxFound{__shfl_xor(x.mx, o, 64), __shfl_xor(x.found, o, 64)};
 }
+struct MaxAt {  // running max and the candidate holding it (at = -1: none seen)
+  float mx;
+  int at;
+};
+__device__ __forceinline__ MaxAt shfl_xor(MaxAt x, int o) {
+  return MaxAt{__shfl_xor(x.mx, o, 64), __shfl_xor(x.at, o, 64)};
+}
 
 // Thread / wave cooperative reduction over the edge ranges [b, e) held by
 // the lanes of one wave.  Lanes whose range is at most `serial_limit` edges

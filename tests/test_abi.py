@@ -57,3 +57,32 @@ def test_grid_removes_exactly_five_percent():
 def test_gnp_average_degree():
     g = gdb.gen_gnp(20000, 8.0, seed=4)
     assert abs(g.m / g.n - 8.0) < 0.3
+
+
+def test_task_keeps_its_device_objects_alive():
+    # make_task copies raw device pointers into the C struct; the Python
+    # objects that own that memory must live as long as the task, or a
+    # launch would read freed memory.  Stand-ins with the same attributes
+    # (no device allocation here).
+    import gc
+    import weakref
+
+    class Buf:
+        def __init__(self, ptr):
+            self.ptr = ptr
+
+    class Graph:
+        def __init__(self):
+            self.n, self.m = 4, 6
+            self.row, self.col, self.w = Buf(16), Buf(32), Buf(48)
+
+    g, out, nv = Graph(), Buf(64), Buf(80)
+    refs = [weakref.ref(x) for x in (g, out, nv)]
+    t = gdb.make_task(gdb.TASK_COLOR, g, out, node_value=nv)
+    assert (t.row, t.col, t.out, t.node_value) == (16, 32, 64, 80)
+    del g, out, nv
+    gc.collect()
+    assert all(r() is not None for r in refs)
+    del t
+    gc.collect()
+    assert all(r() is None for r in refs)

@@ -162,3 +162,52 @@ def test_batch_graph_replays_match_eager(ctx):
     finally:
         graph.close()
 
+
+
+def hub_ring(n_ring, dup):
+    # Vertex 0 links to every ring vertex (a row longer than one lane walks);
+    # the ring vertices link to their two ring neighbours; `dup` repeats
+    # every edge.  Symmetric.
+    n = 1 + n_ring
+    src, dst = [], []
+    for i in range(1, n):
+        j = 1 + (i % n_ring)
+        src += [0, i, i, j]
+        dst += [i, 0, j, i]
+    src = np.repeat(np.array(src, np.int32), dup)
+    dst = np.repeat(np.array(dst, np.int32), dup)
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    row = np.zeros(n + 1, np.int32)
+    np.add.at(row, src + 1, 1)
+    return gdb.CSR(n, np.cumsum(row).astype(np.int32), dst, np.ones(dst.size, np.int32))
+
+
+@pytest.mark.parametrize("team", [1, 4])
+@pytest.mark.parametrize("order", ["index", "reversed", "random"])
+def test_batch_color_blockers(ctx, team, order):
+    # A vertex re-walks its row only when the neighbour holding its maximum
+    # is coloured.  Index-ordered values make grids colour as a wavefront
+    # (every vertex waits on one neighbour for many rounds); reversed and
+    # random orders, duplicate edges and a hub row (reduced by the whole
+    # wave) move the maxima around.
+    gs = [gdb.gen_grid(40, 40, 0.0, seed=1), gdb.gen_grid(23, 61, 0.1, seed=2), hub_ring(100, 1), hub_ring(37, 2),
+          gdb.gen_gnp(3000, 8.0, seed=3)]
+    tasks, outs = [], []
+    for gi, g in enumerate(gs):
+        if order == "index":
+            v = gdb.pannotia_values(g.n)
+        elif order == "reversed":
+            v = gdb.pannotia_values(g.n)[::-1].copy()
+        else:
+            v = gdb.permutation(g.n, 11 + gi).astype(np.float32) / np.float32(g.n + 1)
+        nv = gdb.DeviceBuffer.from_numpy(v)
+        out = gdb.DeviceBuffer.empty(g.n, np.int32)
+        dg = gdb.DeviceCSR(g)
+        tasks.append(gdb.make_task(gdb.TASK_COLOR, dg, out, source=0, node_value=nv))
+        outs.append((g, v, out, nv, dg))
+    rounds, _ = gdb.batch_solve(ctx, tasks, team, gdb.Opts.make(wgs=256, groups=8))
+    for (g, v, out, _, _), r in zip(outs, rounds):
+        want, rr = oracle.color_ref(g.row, g.col, v)
+        assert (out.to_numpy() == want).all()
+        assert r == rr
