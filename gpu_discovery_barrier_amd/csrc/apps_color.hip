@@ -119,6 +119,12 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
   // round 1 (chunks wave + k * nwaves), then the ones whose rows were still
   // uncoloured, compacted in place into the same strided slots of clist.
   int wchunks = nchunks > wave ? (nchunks - wave + nwaves - 1) / nwaves : 0;
+  // Light rows' blockers: the neighbour that held the row's maximum (-1
+  // none).  max_d is the largest value among the uncoloured neighbours, and
+  // neighbours only leave that set, so while the blocker is uncoloured the
+  // maximum stands and the row is not walked; it is walked again once the
+  // blocker is coloured.  The keys equal the full walk's every round.
+  int* blk = mdk + 3 * (size_t)n;
   int r = 1;
   int nbar = 0;
   for (;;) {
@@ -135,12 +141,14 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
     // == -1 && !(have_prev && fkey(nv[u]) > md_prev[u]).
     bool st = false;
     GDB_FOR_UNIFORM(v, valid, tid, stride, n) {
-      bool active = false;
-      int b = 0, e = 0;
+      bool active = false, blocked = false;
+      int b = 0, e = 0, bk = -1, mp = 0;
       if (valid) {
         // All of v's words at once (one memory round trip), then decide.
         const int cv = color[v];
-        const int kv = fkey(nv[v]), mp = md_prev[v];
+        const int kv = fkey(nv[v]);
+        mp = md_prev[v];
+        bk = blk[v];
         const int rb = row[v], re = row[v + 1];
         if (cv == -1) {
           if (have_prev && kv > mp) {
@@ -149,14 +157,16 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
             b = rb;
             e = re;
             active = (e - b) <= kHeavyDeg;  // longer rows: the chunk pass below
+            if (active && bk >= 0)
+              blocked = color[bk] == -1 && !(have_prev && fkey(nv[bk]) > md_prev[bk]);
           }
         }
       }
       const bool deg1 = (e - b) == 1;
       // kSlots edges per lane step: the column loads, then the three
       // gathers per neighbour, then the compares.
-      auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> gdb::MaxFound {
-        if (!gdb::any_slot(ok)) return gdb::MaxFound{-1.f, 0};  // no slot: no loads
+      auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> gdb::MaxAt {
+        if (!gdb::any_slot(ok)) return gdb::MaxAt{-1.f, -1};  // no slot: no loads
         int u[kSlots], cu[kSlots], mu[kSlots];
         float xu[kSlots];
 #pragma unroll
@@ -168,25 +178,28 @@ __global__ __launch_bounds__(WGS) void k_color_tuned(gdb_kernel_ctx_t* ctx, cons
           xu[j] = gdb::ldg(nv + uu);
           mu[j] = gdb::ldg(md_prev + uu);
         }
-        gdb::MaxFound a{-1.f, 0};
+        gdb::MaxAt a{-1.f, -1};
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) {
           const bool unc = ok[j] && cu[j] == -1 && !(have_prev && fkey(xu[j]) > mu[j]);
-          if (unc) {
-            a.mx = xu[j] > a.mx ? xu[j] : a.mx;
-            a.found = 1;
-          }
+          if (unc && (a.at < 0 || xu[j] > a.mx)) a = gdb::MaxAt{xu[j], u[j]};
         }
         return a;
       };
-      auto red = [](gdb::MaxFound a, gdb::MaxFound c) {
-        return gdb::MaxFound{a.mx > c.mx ? a.mx : c.mx, a.found | c.found};
-      };
-      const gdb::MaxFound res =
-          gdb::wave_walk_n<kSlots>(active && !deg1, b, e, gdb::MaxFound{-1.f, 0}, step, red, kHeavyDeg);
+      auto red = [](gdb::MaxAt a, gdb::MaxAt c) { return (c.at >= 0 && (a.at < 0 || c.mx > a.mx)) ? c : a; };
+      const gdb::MaxAt res =
+          gdb::wave_walk_n<kSlots>(active && !deg1 && !blocked, b, e, gdb::MaxAt{-1.f, -1}, step, red, kHeavyDeg);
       if (active) {
-        md_cur[v] = deg1 ? none : fkey(res.mx);
-        if (!deg1 && res.found) st = true;
+        if (blocked) {
+          md_cur[v] = mp;  // the maximum of the previous round stands
+          st = true;
+        } else {
+          md_cur[v] = deg1 ? none : fkey(res.mx > -1.f ? res.mx : -1.f);
+          if (!deg1) {
+            if (res.at != bk) blk[v] = res.at;
+            if (res.at >= 0) st = true;
+          }
+        }
       }
     }
     // Long rows in chunks of kChunkEdges edges (one per lane), spread over
@@ -321,6 +334,7 @@ __global__ void k_color_init(int* color, float* max_d, int* mdk, int* flags, int
       mdk[i] = none;
       mdk[i + n] = none;
       mdk[i + 2 * (size_t)n] = none;
+      mdk[i + 3 * (size_t)n] = -1;  // blockers: none yet
     }
   }
   if (i < 8) flags[i] = 0;
@@ -365,7 +379,7 @@ extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* 
   const int nheavy = (int)heavy.size();
   const int nchunks = (int)chunks.size();
   const size_t bytes = carve_bytes<float>((size_t)2 * n) + carve_bytes<int>(16) +
-                       (tuned_persistent ? carve_bytes<int>((size_t)3 * n) + carve_bytes<int>(nheavy + 1) +
+                       (tuned_persistent ? carve_bytes<int>((size_t)4 * n) + carve_bytes<int>(nheavy + 1) +
                                                carve_bytes<int4>(nchunks + 1) * 2
                                          : 0);
   int rc = ensure_workspace(ctx, bytes);
@@ -373,7 +387,7 @@ extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* 
   Carve cv(ctx->ws);
   float* maxd = cv.take<float>((size_t)2 * n);
   int* flags = cv.take<int>(16);
-  int* mdk = tuned_persistent ? cv.take<int>((size_t)3 * n) : nullptr;
+  int* mdk = tuned_persistent ? cv.take<int>((size_t)4 * n) : nullptr;  // 3 key buffers + blockers
   int* d_heavy = tuned_persistent ? cv.take<int>(nheavy + 1) : nullptr;
   int4* d_chunks = tuned_persistent ? cv.take<int4>(nchunks + 1) : nullptr;
   int4* d_clist = tuned_persistent ? cv.take<int4>(nchunks + 1) : nullptr;

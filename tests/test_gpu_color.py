@@ -90,3 +90,23 @@ def test_color_single_xcd(ctx):
             assert (got == want).all() and st.rounds == rounds
     finally:
         ctx.set_xcd_only(-1)
+
+
+@pytest.mark.parametrize("order", ["index", "reversed"])
+@pytest.mark.parametrize("shape", ["grid", "grid_holes", "rmat"])
+def test_color_tuned_blockers(ctx, shape, order):
+    # The tuned kernel re-walks a light row only when the neighbour holding
+    # its maximum is coloured.  Index-ordered values make a grid colour as a
+    # wavefront (a vertex waits on one neighbour for many rounds); R-MAT mixes
+    # light rows with chunked heavy rows that can be blockers.
+    g = {"grid": lambda: gdb.gen_grid(48, 48, 0.0, seed=1),
+         "grid_holes": lambda: gdb.gen_grid(31, 57, 0.15, seed=2),
+         "rmat": lambda: gdb.gen_rmat(11, 8, seed=5)}[shape]()
+    nv = gdb.pannotia_values(g.n)
+    if order == "reversed":
+        nv = nv[::-1].copy()
+    want, rr = oracle.color_ref(g.row, g.col, nv)
+    got, st = run(ctx, np.asarray(g.row), np.asarray(g.col), nv, wgs=128, groups=16,
+                  schedule=gdb.SCHED_PERSISTENT, variant=gdb.VARIANT_TUNED)
+    assert (got == want).all()
+    assert st.rounds == rr
