@@ -406,8 +406,10 @@ __device__ __forceinline__ int* rec_int(int4* st, size_t rec, int part, int s) {
   return (int*)&st[rec * (2 * kNR) + part * kNR] + s;
 }
 
+// Gathers in flight bound this kernel: it is held to 96 VGPRs (five waves per
+// SIMD instead of four) for a few spilled values.
 template <int WGS>
-__global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
+__global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k_bc_multi(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
                                                   const int* __restrict__ col, int4* st, float* bc,
                                                   const int* sources, int nsrc, int* q, int* stamp,
                                                   int* reach, int* loff, int* tail, int n, int* out) {
