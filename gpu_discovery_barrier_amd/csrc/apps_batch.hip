@@ -819,6 +819,8 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
   return it - 1;
 }
 
+constexpr int kColorVerts = 2;  // vertices per lane per colour pass (4: slower, spills)
+
 // ---- colour on a team (one barrier per round, as k_color_tuned) ------------
 // scratch: pk[n] (int4: colour, value bits, max_d of the even and odd rounds),
 // then blk[n] (the neighbour that set max_d, -1 none).
@@ -861,48 +863,68 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
       return p.x == -1 && !(have_prev && __int_as_float(p.y) > __int_as_float(cur ? p.z : p.w));
     };
     bool st = false;
-    GDB_FOR_UNIFORM(v, valid, tid, stride, n) {
-      bool active = false, blocked = false;
-      int b = 0, e = 0, bk = -1;
-      int4 p = make_int4(0, 0, 0, 0);
-      if (valid) {
-        p = pk[v];
-        bk = blk[v];
-        const float nvv = __int_as_float(p.y);
-        const float mp = __int_as_float(cur ? p.z : p.w);
-        if (p.x == -1) {
-          if (have_prev && nvv > mp) {
-            ((int*)&pk[v])[0] = r - 1;
-          } else {
-            active = true;
-            b = k.row[v];
-            e = k.row[v + 1];
-            if (bk >= 0) blocked = uncoloured(pk[bk]);
-          }
-        }
+    auto edge = [&](int kk) -> gdb::MaxAt {
+      const int u = k.col[kk];
+      const int4 q = pk[u];
+      return uncoloured(q) ? gdb::MaxAt{__int_as_float(q.y), u} : gdb::MaxAt{-1.f, -1};
+    };
+    auto red = [](gdb::MaxAt a, gdb::MaxAt c) { return (c.at >= 0 && (a.at < 0 || c.mx > a.mx)) ? c : a; };
+    // kColorVerts vertices per lane per pass (v, v + stride, ...), their loads
+    // issued together: a blocked vertex is a short chain of dependent loads (its
+    // record and blocker id, then the blocker's record), so several chains
+    // in flight hide more latency.  Loads are branch-free (vertex 0 stands in
+    // for a missing vertex or blocker); the row walks follow one by one.
+    for (long long base = (long long)tid - threadIdx.x; base < (long long)n; base += (long long)kColorVerts * stride) {
+      int vv[kColorVerts], bk[kColorVerts], b[kColorVerts], e[kColorVerts];
+      bool val[kColorVerts], active[kColorVerts], blocked[kColorVerts];
+      int4 p[kColorVerts];
+#pragma unroll
+      for (int j = 0; j < kColorVerts; ++j) {
+        const long long vi = base + threadIdx.x + (long long)j * stride;
+        val[j] = vi < (long long)n;
+        vv[j] = val[j] ? (int)vi : 0;
+        p[j] = pk[vv[j]];
+        bk[j] = blk[vv[j]];
+        b[j] = k.row[vv[j]];
+        e[j] = k.row[vv[j] + 1];
       }
-      const bool deg1 = (e - b) == 1;
-      auto edge = [&](int kk) -> gdb::MaxAt {
-        const int u = k.col[kk];
-        const int4 q = pk[u];
-        return uncoloured(q) ? gdb::MaxAt{__int_as_float(q.y), u} : gdb::MaxAt{-1.f, -1};
-      };
-      auto red = [](gdb::MaxAt a, gdb::MaxAt c) { return (c.at >= 0 && (a.at < 0 || c.mx > a.mx)) ? c : a; };
-      const gdb::MaxAt res = gdb::wave_coop_reduce(active && !deg1 && !blocked, b, e, gdb::MaxAt{-1.f, -1}, edge,
-                                                   red, kSerialLimit);
-      if (active) {
-        if (blocked) {
-          // The maximum of the previous round stands; copy it to this
-          // round's word when they differ.
-          const int prev = cur ? p.z : p.w;
-          if ((cur ? p.w : p.z) != prev) ((int*)&pk[v])[cur ? 3 : 2] = prev;
-          st = true;
-        } else {
-          const float md = deg1 ? -1.f : (res.mx > -1.f ? res.mx : -1.f);
-          ((int*)&pk[v])[cur ? 3 : 2] = __float_as_int(md);
-          if (!deg1) {
-            if (res.at != bk) blk[v] = res.at;
-            if (res.at >= 0) st = true;
+      int4 q[kColorVerts];
+#pragma unroll
+      for (int j = 0; j < kColorVerts; ++j) {
+        active[j] = false;
+        if (val[j] && p[j].x == -1) {
+          if (have_prev && __int_as_float(p[j].y) > __int_as_float(cur ? p[j].z : p[j].w))
+            ((int*)&pk[vv[j]])[0] = r - 1;
+          else
+            active[j] = true;
+        }
+        q[j] = pk[active[j] && bk[j] >= 0 ? bk[j] : 0];
+      }
+#pragma unroll
+      for (int j = 0; j < kColorVerts; ++j) {
+        blocked[j] = active[j] && bk[j] >= 0 && uncoloured(q[j]);
+        if (!active[j]) b[j] = e[j] = 0;
+      }
+#pragma unroll
+      for (int j = 0; j < kColorVerts; ++j) {
+        const bool deg1 = (e[j] - b[j]) == 1;
+        const gdb::MaxAt res = gdb::wave_coop_reduce(active[j] && !deg1 && !blocked[j], b[j], e[j],
+                                                     gdb::MaxAt{-1.f, -1}, edge, red, kSerialLimit);
+        if (active[j]) {
+          const int v = vv[j];
+          if (blocked[j]) {
+            // The maximum of the previous round stands; copy it to this
+            // round's word when they differ.
+            const int prev = cur ? p[j].z : p[j].w;
+            if ((cur ? p[j].w : p[j].z) != prev) ((int*)&pk[v])[cur ? 3 : 2] = prev;
+            st = true;
+          } else {
+            const float md = deg1 ? -1.f : (res.mx > -1.f ? res.mx : -1.f);
+            ((int*)&pk[v])[cur ? 3 : 2] = __float_as_int(md);
+            if (!deg1) {
+              if (res.at != bk[j]) blk[v] = res.at;
+              if (res.at >= 0) st = true;
+            }
           }
         }
       }
