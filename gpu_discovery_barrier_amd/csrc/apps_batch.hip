@@ -905,11 +905,46 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
         blocked[j] = active[j] && bk[j] >= 0 && uncoloured(q[j]);
         if (!active[j]) b[j] = e[j] = 0;
       }
+      // Light rows of all kColorVerts vertices walked together, two edges
+      // of each per step, every load branch-free (edge 0 and vertex 0 stand
+      // in for unused slots; a slot is used only when the graph has edges).
+      bool walk[kColorVerts], heavy[kColorVerts];
+      int len[kColorVerts];
+      gdb::MaxAt acc[kColorVerts];
+      int maxlen = 0;
+#pragma unroll
+      for (int j = 0; j < kColorVerts; ++j) {
+        walk[j] = active[j] && (e[j] - b[j]) != 1 && !blocked[j];
+        heavy[j] = walk[j] && (e[j] - b[j]) > kSerialLimit;
+        len[j] = walk[j] && !heavy[j] ? e[j] - b[j] : 0;
+        maxlen = max(maxlen, len[j]);
+        acc[j] = gdb::MaxAt{-1.f, -1};
+      }
+      for (int t = 0; t < maxlen; t += 2) {
+        int u[kColorVerts][2];
+        int4 qq[kColorVerts][2];
+#pragma unroll
+        for (int j = 0; j < kColorVerts; ++j)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) u[j][h] = k.col[t + h < len[j] ? b[j] + t + h : 0];
+#pragma unroll
+        for (int j = 0; j < kColorVerts; ++j)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) qq[j][h] = pk[t + h < len[j] ? u[j][h] : 0];
+#pragma unroll
+        for (int j = 0; j < kColorVerts; ++j)
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+            if (t + h < len[j] && uncoloured(qq[j][h]))
+              acc[j] = red(acc[j], gdb::MaxAt{__int_as_float(qq[j][h].y), u[j][h]});
+      }
 #pragma unroll
       for (int j = 0; j < kColorVerts; ++j) {
         const bool deg1 = (e[j] - b[j]) == 1;
-        const gdb::MaxAt res = gdb::wave_coop_reduce(active[j] && !deg1 && !blocked[j], b[j], e[j],
-                                                     gdb::MaxAt{-1.f, -1}, edge, red, kSerialLimit);
+        // Rows longer than kSerialLimit: the whole wave walks them.
+        const gdb::MaxAt hres =
+            gdb::wave_coop_reduce(heavy[j], b[j], e[j], gdb::MaxAt{-1.f, -1}, edge, red, kSerialLimit);
+        const gdb::MaxAt res = red(acc[j], hres);
         if (active[j]) {
           const int v = vv[j];
           if (blocked[j]) {
