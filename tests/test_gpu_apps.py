@@ -229,3 +229,34 @@ def test_bc_symmetric_walk_matches_two_walks(ctx):
     b, _ = gdb.bc(ctx, d1, d2, src, o)
     bc_close(a, oracle.bc_brandes_tb(g.row, g.col, src))
     bc_close(b, oracle.bc_brandes_tb(g.row, g.col, src))
+
+
+def symmetric_multigraph(scale, seed):
+    # An R-MAT graph with every fifth undirected edge doubled (both arcs) and
+    # a self-loop on every seventh vertex; still symmetric.
+    g = gdb.gen_rmat(scale, 16, seed=seed)
+    src = np.repeat(np.arange(g.n, dtype=np.int32), np.diff(g.row))
+    dst = np.asarray(g.col, np.int32)
+    fwd = np.flatnonzero(src < dst)[::5]
+    loops = np.arange(0, g.n, 7, dtype=np.int32)
+    s = np.concatenate([src, src[fwd], dst[fwd], loops])
+    d = np.concatenate([dst, dst[fwd], src[fwd], loops])
+    order = np.lexsort((d, s))
+    s, d = s[order], d[order]
+    row = np.zeros(g.n + 1, np.int32)
+    np.add.at(row, s + 1, 1)
+    return np.cumsum(row).astype(np.int32), d.astype(np.int32)
+
+
+def test_bc_multigraph_duplicate_edges_and_self_loops(ctx):
+    # Parallel edges count once per edge in sigma and delta (as in the
+    # oracle); a self-loop never joins a level.  Eleven sources: a full
+    # eight-source sweep and a partial one, one source with a self-loop.
+    row, col = symmetric_multigraph(11, 4)
+    n = row.size - 1
+    src = np.concatenate([[0, 7], gdb.permutation(n, 4)[:9]]).astype(np.int32)
+    want = oracle.bc_brandes_tb(row, col, src)
+    d = dev(row, col)
+    for sched in SCHEDS[:3]:
+        got, _ = gdb.bc(ctx, d, d, src, gdb.Opts.make(**sched))
+        bc_close(got, want)
