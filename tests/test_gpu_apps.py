@@ -260,3 +260,31 @@ def test_bc_multigraph_duplicate_edges_and_self_loops(ctx):
     for sched in SCHEDS[:3]:
         got, _ = gdb.bc(ctx, d, d, src, gdb.Opts.make(**sched))
         bc_close(got, want)
+
+
+@pytest.mark.parametrize("sched", SCHEDS[:3], ids=SID)
+def test_worklist_multigraph_parallel_edges_with_different_weights(ctx, sched):
+    # Parallel edges carry different weights, so two slots of one lane can
+    # relax the same vertex with different candidates in one step.
+    row, col = symmetric_multigraph(11, 6)
+    w = (np.arange(col.size, dtype=np.int64) * 7919 % 97 + 1).astype(np.int32)
+    d = dev(row, col, w)
+    for source in (0, 5, 1000):
+        got_b, _ = gdb.bfs_worklist(ctx, d, source, gdb.Opts.make(**sched))
+        assert (got_b == oracle.bfs_tb(row, col, source)).all()
+        got_s, _ = gdb.sssp_worklist(ctx, d, source, gdb.Opts.make(**sched))
+        assert (got_s.astype(np.int64) == oracle.dijkstra_tb(row, col, w, source)).all()
+
+
+@pytest.mark.parametrize("hops", [0, 8])
+def test_batch_multigraph_parallel_edges_with_different_weights(ctx, hops):
+    row, col = symmetric_multigraph(11, 6)
+    w = (np.arange(col.size, dtype=np.int64) * 7919 % 97 + 1).astype(np.int32)
+    d = dev(row, col, w)
+    ob = gdb.DeviceBuffer.empty(row.size - 1, np.uint32)
+    os_ = gdb.DeviceBuffer.empty(row.size - 1, np.uint32)
+    tasks = [gdb.make_task(gdb.TASK_BFS, d, ob, 3, hops=hops),
+             gdb.make_task(gdb.TASK_SSSP, d, os_, 3, delta=40, hops=hops)]
+    gdb.batch_solve(ctx, tasks, 2, gdb.Opts.make(wgs=128, groups=4))
+    assert (ob.to_numpy() == oracle.bfs_tb(row, col, 3)).all()
+    assert (os_.to_numpy().astype(np.int64) == oracle.dijkstra_tb(row, col, w, 3)).all()
