@@ -94,6 +94,8 @@ class Task(C.Structure):
         ("rounds", C.c_int),
         ("time_us", C.c_double),
         ("arm_iters", C.c_int * 4),
+        ("work_items", C.c_longlong),
+        ("work_edges", C.c_longlong),
     ]
 
 
@@ -153,6 +155,9 @@ def _declare(lib) -> None:
         "gdb_number_of_participating_groups": (I, [P]),
         "gdb_ctx_device_error": (I, [P]),
         "gdb_ctx_stream": (P, [P]),
+        "gdb_ctx_device_ptr": (I, [P, C.POINTER(P)]),
+        "gdb_launch_on_ctx": (I, [P, P, C.POINTER(P), I, I, C.c_size_t]),
+        "gdb_time_protocol_kernel": (I, [P, P, C.POINTER(P), I, I, C.c_size_t, C.POINTER(C.c_double)]),
         "gdb_time_protocol": (I, [P, I, I, I, C.POINTER(C.c_double)]),
         "gdb_occupancy_test": (I, [P, I, I, I, I, C.POINTER(I)]),
         "gdb_max_resident_groups": (I, [I, I, C.POINTER(I)]),
@@ -580,6 +585,8 @@ def make_selector(mode: int = SEL_NONE, fixed_arm: int = 0, narrow_groups: int =
                   weights: "DeviceBuffer | None" = None, trace: "DeviceBuffer | None" = None,
                   trace_count: "DeviceBuffer | None" = None) -> Selector:
     s = Selector()
+    # The structure holds raw device pointers; keep their owners alive with it.
+    s._keep = (weights, trace, trace_count)
     s.mode, s.fixed_arm, s.narrow_groups, s.seed = mode, fixed_arm, narrow_groups, seed & 0xFFFFFFFF
     s.weights = weights.ptr if weights is not None else None
     if trace is not None:
@@ -607,6 +614,7 @@ def batch_solve(ctx: Context, tasks: list, team_groups: int = 1, opts=None, sele
     for i, t in enumerate(tasks):
         t.rounds, t.time_us = arr[i].rounds, arr[i].time_us
         t.arm_iters[:] = arr[i].arm_iters[:]
+        t.work_items, t.work_edges = arr[i].work_items, arr[i].work_edges
     return [arr[i].rounds for i in range(len(tasks))], st
 
 
@@ -618,6 +626,9 @@ class BatchGraph:
     def __init__(self, ctx: Context, tasks: list, team_groups: int = 1, opts=None,
                  selector: "Selector | None" = None):
         self.tasks = tasks
+        # The recorded kernel arguments point into the tasks' and the
+        # selector's device buffers: both stay referenced by the graph.
+        self.selector = selector
         self.arr = (Task * len(tasks))(*tasks)
         self.h = P()
         o = _opts(opts)
@@ -631,6 +642,7 @@ class BatchGraph:
         for i, t in enumerate(self.tasks):
             t.rounds, t.time_us = self.arr[i].rounds, self.arr[i].time_us
             t.arm_iters[:] = self.arr[i].arm_iters[:]
+            t.work_items, t.work_edges = self.arr[i].work_items, self.arr[i].work_edges
         return [self.arr[i].rounds for i in range(len(self.tasks))], st
 
     def close(self) -> None:

@@ -1198,8 +1198,10 @@ int collect_batch(const BatchPlan& P, gdb_task* tasks, gdb_stats* st) {
     // Worklist tasks report items / edges processed; colour tasks count the
     // graph once.
     const bool wl = tk.algo != GDB_TASK_COLOR;
-    st->work_items += wl ? (long long)wk[0] : tk.n;
-    st->work_edges += wl ? (long long)wk[1] : tk.m;
+    tk.work_items = wl ? (long long)wk[0] : tk.n;
+    tk.work_edges = wl ? (long long)wk[1] : tk.m;
+    st->work_items += tk.work_items;
+    st->work_edges += tk.work_edges;
     // Device wall clock (100 MHz) from the team's start to end of the task.
     tk.time_us = (double)(wk[3] - wk[2]) / 100.0;
     for (int a = 0; a < 4; ++a) tk.arm_iters[a] = (int)wk[4 + a];

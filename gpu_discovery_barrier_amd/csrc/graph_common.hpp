@@ -1,7 +1,7 @@
 // Wave64 helpers shared by the graph kernels.
 #pragma once
 #include <hip/hip_runtime.h>
-#include "discovery.hpp"
+#include "gdb_discovery.hpp"
 
 namespace gdb {
 

@@ -7,7 +7,7 @@
 #include <cstddef>
 #include <cstring>
 #include <string>
-#include "discovery.hpp"
+#include "gdb_discovery.hpp"
 #include "runtime.hpp"
 
 // ---------------------------------------------------------------------------
@@ -44,6 +44,12 @@ __global__ void gdb_k_fill_i32(int* p, int value, long long count) {
 
 // Clears the sticky error word before a solve.
 __global__ void gdb_k_clear_error(gdb_kernel_ctx_t* ctx) { gdb::st_rlx(&ctx->error.v, GDB_OK); }
+
+// Rewrites the wait bounds of a context without resetting its other words.
+__global__ void gdb_k_set_timeouts(gdb_kernel_ctx_t* ctx, int quiet_ticks, int timeout_ticks) {
+  gdb::st_rlx(&ctx->quiet_ticks.v, quiet_ticks);
+  gdb::st_rlx(&ctx->timeout_ticks.v, timeout_ticks);
+}
 
 // run_prot (occupancy_test.cl:15-17): protocol only.
 __global__ void gdb_k_run_prot(gdb_kernel_ctx_t* ctx) {
@@ -129,11 +135,19 @@ int fill_i32(gdb_ctx* ctx, int* p, int value, long long count) {
   return GDB_SUCCESS;
 }
 
+static int init_ctx(gdb_ctx* ctx, int skip);
+
 int check_device_error(gdb_ctx* ctx) {
   int err = 0;
   GDB_HIP(hipMemcpyAsync(&err, &ctx->d->error.v, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   GDB_HIP(hipStreamSynchronize(ctx->stream));
-  if (err == GDB_ERR_BARRIER_TIMEOUT || err == GDB_ERR_DISCOVERY_TIMEOUT) return GDB_E_TIMEOUT;
+  if (err == GDB_ERR_BARRIER_TIMEOUT || err == GDB_ERR_DISCOVERY_TIMEOUT) {
+    // A wait that gave up leaves arrival counters, XF flags or protocol
+    // words part-way through an episode; the next launch would start from
+    // them.  Reset the whole context (same configuration) before reporting.
+    int rc = init_ctx(ctx, ctx->skip);
+    return rc ? rc : GDB_E_TIMEOUT;
+  }
   if (err == GDB_ERR_QUEUE_OVERFLOW) return GDB_E_OVERFLOW;
   return GDB_SUCCESS;
 }
@@ -319,6 +333,11 @@ int gdb_ctx_set_timeouts(gdb_ctx* ctx, double wait_us, double quiet_us) {
   if (w > 2.0e9 || q > 2.0e9) return GDB_E_INVALID;
   ctx->timeout_ticks = (unsigned)(wait_us == 0 ? 0 : (w < 1 ? 1 : w));
   ctx->quiet_ticks = (unsigned)q;
+  // The bounds also apply to the next launch without a re-init.
+  hipLaunchKernelGGL(gdb_k_set_timeouts, dim3(1), dim3(1), 0, ctx->stream, ctx->d, (int)ctx->quiet_ticks,
+                     (int)ctx->timeout_ticks);
+  GDB_HIP(hipGetLastError());
+  GDB_HIP(hipStreamSynchronize(ctx->stream));
   return GDB_SUCCESS;
 }
 
@@ -345,6 +364,36 @@ int gdb_ctx_device_error(gdb_ctx* ctx) {
 }
 
 void* gdb_ctx_stream(gdb_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int gdb_ctx_device_ptr(gdb_ctx* ctx, void** out) {
+  if (!ctx || !out) return GDB_E_INVALID;
+  *out = (void*)ctx->d;
+  return GDB_SUCCESS;
+}
+
+int gdb_launch_on_ctx(gdb_ctx* ctx, const void* kernel, void** args, int groups, int wgs, size_t dyn_lds) {
+  if (!ctx || !kernel || groups < 1 || wgs < 1 || wgs > 1024) return GDB_E_INVALID;
+  int rc = prepare_launch(ctx);
+  if (rc) return rc;
+  GDB_HIP(hipLaunchKernel(kernel, dim3(groups), dim3(wgs), args, dyn_lds, ctx->stream));
+  GDB_HIP(hipStreamSynchronize(ctx->stream));
+  return check_device_error(ctx);
+}
+
+int gdb_time_protocol_kernel(gdb_ctx* ctx, const void* kernel, void** args, int upper_bound, int wgs,
+                             size_t dyn_lds, double* ns) {
+  if (!ctx || !kernel || upper_bound < 1 || wgs < 1 || wgs > 1024) return GDB_E_INVALID;
+  int rc = init_ctx(ctx, 0);  // discovery.h:74
+  if (rc) return rc;
+  GDB_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  GDB_HIP(hipLaunchKernel(kernel, dim3(upper_bound), dim3(wgs), args, dyn_lds, ctx->stream));
+  GDB_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  GDB_HIP(hipEventSynchronize(ctx->ev1));
+  float f = 0.f;
+  GDB_HIP(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+  if (ns) *ns = (double)f * 1.0e6;
+  return check_device_error(ctx);
+}
 
 int gdb_max_resident_groups(int wgs, int lms, int* out) {
   if (!out || wgs < 1 || wgs > 1024 || lms < 0) return GDB_E_INVALID;

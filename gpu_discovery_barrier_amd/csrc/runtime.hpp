@@ -4,7 +4,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <vector>
-#include "ctx_layout.h"
+#include "gdb_ctx_layout.h"
 #include "../../include/gdb_amd.h"
 
 struct gdb_ctx {

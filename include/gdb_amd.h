@@ -15,7 +15,11 @@
  *   gdb_ctx_init_skip          init_discovery_kernel_ctx_skip  (discovery.h:36-59)
  *   gdb_number_of_participating_groups
  *                              number_of_participating_groups  (discovery.h:63-68)
- *   gdb_time_protocol          time_protocol                   (discovery.h:71-105)
+ *   gdb_time_protocol_kernel   time_protocol                   (discovery.h:71-105)
+ *   gdb_time_protocol          time_protocol on run_prot (occupancy_test.cl:15-17)
+ *   gdb_ctx_device_ptr         the context cl_mem handed to kernels (color_gb.cpp:182-185,248)
+ *   gdb_launch_on_ctx          clEnqueueNDRangeKernel of a user kernel on the context
+ *                              (e.g. lonestar bfs_port.cpp:139-191)
  *   gdb_occupancy_test         occupancy_test main + run_test kernel
  *        (code/experiments/occupancy_tests/src/occupancy_test.c:21-89,
  *         kernels/occupancy_test.cl:8-11)
@@ -35,6 +39,8 @@
  *   gdb_sssp_worklist          sssp-port drelax2 + dverifysolution
  *        (lonestar_gpu_opencl/apps/sssp/kernel/sssp_port_kernel.cl:42-197,
  *         sssp_common.cl:41-81; sssp_port.cpp:65-178)
+ * Device side (kernels that include gdb_discovery.hpp): DISCOVERY_PROTOCOL,
+ * discovery_barrier, p_get_*, is_participating (discovery.cl:106-226).
  */
 #ifndef GDB_AMD_H
 #define GDB_AMD_H
@@ -62,7 +68,7 @@ extern "C" {
 
 #define GDB_BARRIER_XF 0       /* master/slave flags, discovery.cl:129-190    */
 #define GDB_BARRIER_COUNTER 1  /* central sense-reversing counter             */
-#define GDB_BARRIER_TREE 2     /* per-XCD arrival, one L2 write-back per XCD  */
+#define GDB_BARRIER_TREE 2     /* two-level counter: 16 clusters, then their leaders */
 #define GDB_BARRIER_AUTO 3     /* COUNTER below 64 groups, TREE above (default) */
 
 #define GDB_SCHED_PERSISTENT 0 /* one launch, inter-workgroup barriers        */
@@ -122,8 +128,36 @@ int gdb_number_of_participating_groups(gdb_ctx* ctx);
 int gdb_ctx_device_error(gdb_ctx* ctx);
 void* gdb_ctx_stream(gdb_ctx* ctx);
 
-/* Launches `upper_bound` groups of `wgs` threads (with `lms` bytes of LDS)
- * that only run the protocol; returns its device time in ns. */
+/* The context's device buffer (a gdb_kernel_ctx_t*, include/gdb_ctx_layout.h),
+ * to pass to a caller's own persistent kernel that includes
+ * gdb_discovery.hpp and opens with DISCOVERY_PROTOCOL(ctx).  It is the
+ * cl_mem the reference's hosts create and hand to their kernels
+ * (pannotia/graph_app/color/color_gb.cpp:182-185, clSetKernelArg at :248).
+ * The buffer lives until gdb_ctx_destroy. */
+int gdb_ctx_device_ptr(gdb_ctx* ctx, void** out);
+
+/* Launches a caller's kernel (a __global__ function's host symbol, as taken
+ * by hipLaunchKernel; `args` as for hipLaunchKernel) with `groups` groups of
+ * `wgs` threads on the context's stream, waits, and reports the device
+ * error word (GDB_E_TIMEOUT when a barrier wait gave up).  Clears the error
+ * word first; does not re-initialise the context, which resets itself after
+ * every launch (discovery.cl:67-84), as the reference's apps rely on
+ * (lonestar bfs_port.cpp:139-191 launches twice after one init). */
+int gdb_launch_on_ctx(gdb_ctx* ctx, const void* kernel, void** args, int groups, int wgs, size_t dyn_lds);
+
+/* time_protocol (discovery.h:71-105): initialises the context, then times
+ * one launch of the CALLER's kernel with `upper_bound` groups of `wgs`
+ * threads (`dyn_lds` bytes of dynamic LDS) on the context's stream; *ns is
+ * the device time.  The kernel's arguments must include the context's
+ * device pointer (gdb_ctx_device_ptr), as the reference's caller sets
+ * argument 0 to the context buffer (occupancy_tests/src/time_prot.c:57-62). */
+int gdb_time_protocol_kernel(gdb_ctx* ctx, const void* kernel, void** args, int upper_bound, int wgs,
+                             size_t dyn_lds, double* ns);
+
+/* time_protocol with the library's own protocol-only kernel (run_prot,
+ * occupancy_tests/kernels/occupancy_test.cl:15-17) as the caller's kernel:
+ * `upper_bound` groups of `wgs` threads with `lms` bytes of LDS; *ns is its
+ * device time. */
 int gdb_time_protocol(gdb_ctx* ctx, int upper_bound, int wgs, int lms, double* ns);
 /* Protocol (prot=1) or skip mode (prot=0) followed by one barrier, with wgc
  * groups of wgs threads and lms bytes of LDS.  Skip-mode launches beyond the
@@ -217,6 +251,9 @@ typedef struct {
   double time_us;           /* out: device time from task start to task end */
   int arm_iters[4];         /* out: relaxing iterations (non-empty frontier) run with each
                                schedule arm (GDB_ARM_*) */
+  long long work_items;     /* out: BFS / SSSP worklist items processed (duplicates included);
+                               colour: n */
+  long long work_edges;     /* out: BFS / SSSP edges examined; colour: m */
 } gdb_task;
 
 /* team_groups: groups per team (0 = 1).  opts->wgs and opts->groups pick the

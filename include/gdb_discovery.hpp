@@ -1,5 +1,22 @@
 // Device-side occupancy-discovery protocol, execution-environment queries
-// and inter-workgroup barriers for gfx950 (HIP).
+// and inter-workgroup barriers for gfx950 (HIP).  Public header: a caller's
+// persistent kernel includes it (as the reference's kernels include
+// discovery.cl), takes the context's device pointer (gdb_ctx_device_ptr in
+// gdb_amd.h) as an argument, and is launched with gdb_launch_on_ctx /
+// gdb_time_protocol_kernel or on the context's stream by the caller:
+//
+//   #include "gdb_discovery.hpp"
+//   __global__ void my_kernel(gdb_kernel_ctx_t* ctx, int* data) {
+//     DISCOVERY_PROTOCOL(ctx);                 // non-participants return here
+//     for (int i = p_get_global_id(ctx, &local_ctx); ...;
+//          i += p_get_global_size(ctx, &local_ctx)) ...
+//     if (!discovery_barrier(ctx, &local_ctx)) return;  // false: a wait gave up
+//   }
+//
+// Every participating group must call discovery_barrier the same number of
+// times with all its threads (it contains workgroup barriers), as in the
+// reference.  The header only needs hip_runtime.h; code built from it with
+// hipcc --offload-arch=gfx950 shares the context with libgdb_amd.so.
 //
 // Reference API kept (code/discovery_protocol/api/discovery.cl):
 //   DISCOVERY_PROTOCOL(ctx)       discovery.cl:213-226 (non-participants return)
@@ -27,7 +44,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <climits>
-#include "ctx_layout.h"
+#include "gdb_ctx_layout.h"
 
 #define GDB_POLL_CLOSED (1 << 30)
 #define GDB_POLL_MASK (GDB_POLL_CLOSED - 1)

@@ -23,7 +23,7 @@ I = C.c_int
 
 
 def build() -> Path:
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE)], check=True)
     return LIB_PATH
 
 
@@ -47,6 +47,7 @@ def lib():
             "orc_verify_distances_ref": (C.c_longlong, [I, P, P, P, P]),
             "orc_bc_brandes_tb": (I, [I, P, P, P, I, P]),
             "orc_bc_pairs_tb": (I, [I, P, P, P, I, P]),
+            "orc_bc_brandes_mt": (I, [I, P, P, P, I, I, P]),
             "orc_bc_ref_f32": (I, [I, P, P, P, P, P, I, P]),
             "orc_block_scan_ref": (I, [I, P, P]),
             "orc_dp_model": (I, [I, I, I, I, I, I, I, C.POINTER(I), P, C.POINTER(C.c_longlong)]),
@@ -165,6 +166,19 @@ def bc_brandes_tb(row, col, sources):
     n = row.size - 1
     out = np.empty(n, np.float64)
     lib().orc_bc_brandes_tb(n, _p(row), _p(col), _p(sources), sources.size, _p(out))
+    return out
+
+
+def bc_brandes_mt(row, col, sources, threads=None):
+    """bc_brandes_tb with the sources spread over `threads` host threads."""
+    import os
+    row, col, sources = _i32(row), _i32(col), _i32(sources)
+    threads = threads or min(16, os.cpu_count() or 1)
+    n = row.size - 1
+    out = np.empty(n, np.float64)
+    rc = lib().orc_bc_brandes_mt(n, _p(row), _p(col), _p(sources), sources.size, int(threads), _p(out))
+    if rc:
+        raise MemoryError("orc_bc_brandes_mt")
     return out
 
 

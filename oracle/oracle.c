@@ -451,6 +451,61 @@ int orc_bc_brandes_tb(int n, const int* row, const int* col, const int* sources,
   return 0;
 }
 
+/* orc_bc_brandes_tb split over threads: thread t takes sources t, t + T, ...
+ * into its own FP64 accumulator; the accumulators are summed in thread order,
+ * so the result does not depend on scheduling. */
+typedef struct {
+  int n, nsrc, t, T, rc;
+  const int *row, *col, *sources;
+  double* acc;
+} bc_mt_arg;
+
+static void* bc_mt_worker(void* p) {
+  bc_mt_arg* a = (bc_mt_arg*)p;
+  int cnt = 0;
+  for (int i = a->t; i < a->nsrc; i += a->T) ++cnt;
+  int* mine = (int*)malloc(sizeof(int) * (size_t)(cnt > 0 ? cnt : 1));
+  if (!mine) {
+    a->rc = -2;
+    return NULL;
+  }
+  cnt = 0;
+  for (int i = a->t; i < a->nsrc; i += a->T) mine[cnt++] = a->sources[i];
+  a->rc = orc_bc_brandes_tb(a->n, a->row, a->col, mine, cnt, a->acc);
+  free(mine);
+  return NULL;
+}
+
+int orc_bc_brandes_mt(int n, const int* row, const int* col, const int* sources, int nsrc, int threads,
+                      double* bc) {
+  if (threads < 1) threads = 1;
+  if (threads > nsrc) threads = nsrc > 0 ? nsrc : 1;
+  const size_t sz = (size_t)(n > 0 ? n : 1);
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+  bc_mt_arg* args = (bc_mt_arg*)calloc((size_t)threads, sizeof(bc_mt_arg));
+  double* acc = (double*)malloc(sizeof(double) * sz * (size_t)threads);
+  if (!th || !args || !acc) return -2;
+  for (int t = 0; t < threads; ++t) {
+    bc_mt_arg a = {n, nsrc, t, threads, 0, row, col, sources, acc + sz * (size_t)t};
+    args[t] = a;
+    pthread_create(&th[t], NULL, bc_mt_worker, &args[t]);
+  }
+  int rc = 0;
+  for (int t = 0; t < threads; ++t) {
+    pthread_join(th[t], NULL);
+    if (args[t].rc) rc = args[t].rc;
+  }
+  for (int i = 0; i < n; ++i) {
+    double v = 0.0;
+    for (int t = 0; t < threads; ++t) v += acc[sz * (size_t)t + (size_t)i];
+    bc[i] = v;
+  }
+  free(th);
+  free(args);
+  free(acc);
+  return rc;
+}
+
 static void bfs_counts(int n, const int* row, const int* col, int s, int* d, double* sigma, int* q) {
   for (int i = 0; i < n; ++i) {
     d[i] = -1;

@@ -62,6 +62,10 @@ long long orc_verify_distances_ref(int n, const int* row, const int* col, const 
  * pairs, no 1/2 factor, bc[s] not credited for its own source. */
 int orc_bc_brandes_tb(int n, const int* row, const int* col, const int* sources, int nsrc,
                       double* bc);
+/* orc_bc_brandes_tb with the sources dealt to `threads` pthreads (each with
+ * its own FP64 accumulator, summed in thread order). */
+int orc_bc_brandes_mt(int n, const int* row, const int* col, const int* sources, int nsrc, int threads,
+                      double* bc);
 /* Pair-dependency definition: sum over s in sources, t != s, v != s,t of
  * sigma_sv * sigma_vt / sigma_st when d(s,v) + d(v,t) = d(s,t).  O(n^2 m). */
 int orc_bc_pairs_tb(int n, const int* row, const int* col, const int* sources, int nsrc,

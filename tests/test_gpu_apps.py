@@ -2,8 +2,9 @@
 for every schedule, through the C ABI.
 
 Bars: MIS sets, SSSP distances and BFS levels bit-exact; BC (FP32) within
-1e-4 relative of the FP64 Brandes oracle, with an absolute floor of 1e-4 times
-the largest value for entries near zero."""
+1e-4 relative of the FP64 Brandes oracle entry by entry, |got - want| <=
+1e-4 * |want|, with an absolute floor of 1e-6 that only matters for entries
+that are (near) zero."""
 import numpy as np
 import pytest
 
@@ -22,6 +23,7 @@ SCHEDS = [
 ]
 SID = lambda s: "-".join(map(str, s.values()))
 BC_RTOL = 1e-4
+BC_ATOL = 1e-6
 
 
 @pytest.fixture(scope="module")
@@ -37,11 +39,13 @@ def dev(row, col, w=None):
 
 
 def bc_close(got, want):
+    """Per-entry relative bar: |got - want| <= 1e-4 |want|, floor 1e-6."""
     want = np.asarray(want, np.float64)
-    floor = 1e-4 * max(1.0, float(np.abs(want).max()))
     err = np.abs(got.astype(np.float64) - want)
-    ok = err <= np.maximum(BC_RTOL * np.abs(want), floor)
+    ok = err <= np.maximum(BC_RTOL * np.abs(want), BC_ATOL)
     assert ok.all(), (np.flatnonzero(~ok)[:10], got[~ok][:10], want[~ok][:10])
+    nz = want != 0
+    return float((err[nz] / np.abs(want[nz])).max()) if nz.any() else 0.0
 
 
 # ---------------------------------------------------------------- MIS
@@ -116,6 +120,40 @@ def test_bc_rmat(ctx, sched):
     got, st = gdb.bc(ctx, d, d, src, gdb.Opts.make(**sched))
     print(SID(sched), st.as_dict())
     bc_close(got, want)
+
+
+@pytest.mark.parametrize("sched", SCHEDS, ids=SID)
+def test_bc_rmat15_64_sources(ctx, sched):
+    # R-MAT 15 with 64 sources: eight full eight-source sweeps on the tuned
+    # schedule, 64 single-source passes on the others.
+    g = gdb.gen_rmat(15, 16, seed=3)
+    deg = np.diff(g.row)
+    src = gdb.permutation(g.n, 15)
+    src = src[deg[src] > 0][:64].astype(np.int32)
+    want = oracle.bc_brandes_tb(g.row, g.col, src)
+    d = dev(g.row, g.col)
+    got, st = gdb.bc(ctx, d, d, src, gdb.Opts.make(**sched))
+    rel = bc_close(got, want)
+    print(SID(sched), st.as_dict(), "max rel", rel)
+    assert (want > 0).sum() > g.n // 4  # most vertices carry a nonzero centrality
+
+
+def test_bc_close_rejects_small_entries_off():
+    # The bar must catch a wrong small entry: zeroing the smallest nonzero
+    # oracle value fails it, as does a 2e-4 relative error on one entry.
+    g = gdb.gen_rmat(10, 16, seed=3)
+    src = gdb.permutation(g.n, 3)[:8]
+    want = oracle.bc_brandes_tb(g.row, g.col, src)
+    k = int(np.argmin(np.where(want > 0, want, np.inf)))
+    bad = want.astype(np.float32).copy()
+    bad[k] = 0
+    with pytest.raises(AssertionError):
+        bc_close(bad, want)
+    j = int(np.argmax(want))
+    bad = want.astype(np.float32).copy()
+    bad[j] *= 1 + 2e-4
+    with pytest.raises(AssertionError):
+        bc_close(bad, want)
 
 
 def test_bc_sweeps_partial_batch_and_repeated_sources(ctx):

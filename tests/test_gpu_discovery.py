@@ -129,6 +129,32 @@ def test_timeout_ends_wait_instead_of_hanging(ctx):
     assert p >= 1
 
 
+@pytest.mark.parametrize("bar", [gdb.BARRIER_XF, gdb.BARRIER_COUNTER, gdb.BARRIER_TREE])
+def test_solve_after_forced_timeout_matches_oracle(bar):
+    # A zero wait bound (set from the host) makes the first barrier give up
+    # part-way, leaving arrival counts / flags behind.  With the bound
+    # restored and no explicit init, the next solve on the same context must
+    # still be exact: the library resets the context when it reports the
+    # timeout.
+    import numpy as np
+    import oracle
+
+    g = gdb.gen_grid(96, 96, 0.05, seed=2, wlo=1, whi=255)
+    dg = gdb.DeviceCSR(g)
+    want = oracle.dijkstra_tb(g.row, g.col, g.w, 0)
+    with gdb.Context(bar_kind=bar) as c:
+        o = gdb.Opts.make(variant=gdb.VARIANT_TUNED, wgs=256, groups=96)
+        c.set_timeouts(0.0, 20.0)
+        with pytest.raises(gdb.GdbError) as e:
+            gdb.sssp_worklist(c, dg, 0, o)
+        assert e.value.code == gdb.GDB_E_TIMEOUT
+        c.set_timeouts(10e6, 20.0)
+        for _ in range(2):
+            got, st = gdb.sssp_worklist(c, dg, 0, o)
+            assert (got.astype(np.int64) == want).all()
+            assert st.participating > 1
+
+
 def test_time_protocol(ctx):
     ns = ctx.time_protocol(1000, 256)
     print("protocol ns", ns, "P", ctx.participating())

@@ -167,6 +167,21 @@ def test_bc_brandes_matches_pair_definition():
     np.testing.assert_allclose(f, a, rtol=1e-5, atol=1e-4)
 
 
+def test_bc_brandes_threaded_matches_pair_definition():
+    # The threaded form (sources dealt to threads, accumulators summed in
+    # thread order) against the pair-dependency definition and the
+    # single-thread form, with more threads than sources and uneven shares.
+    g = gdb.gen_gnp(60, 4.0, seed=12)
+    src = np.arange(1, 60, 4, dtype=np.int32)
+    want = oracle.bc_pairs_tb(g.row, g.col, src)
+    for t in (1, 3, 7, 64):
+        np.testing.assert_allclose(oracle.bc_brandes_mt(g.row, g.col, src, threads=t), want, rtol=1e-12, atol=1e-9)
+    r = gdb.gen_rmat(11, 16, seed=4)
+    s = gdb.permutation(r.n, 4)[:21]
+    np.testing.assert_allclose(oracle.bc_brandes_mt(r.row, r.col, s, threads=5),
+                               oracle.bc_brandes_tb(r.row, r.col, s), rtol=1e-12, atol=1e-9)
+
+
 def test_bc_path_closed_form():
     n = 9
     row, col, _ = csr_from_edges(n, [(i, i + 1) for i in range(n - 1)])
