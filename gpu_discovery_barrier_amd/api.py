@@ -181,6 +181,7 @@ def _declare(lib) -> None:
         "gdb_bc": (I, [P, I, I, P, P, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_bfs_worklist": (I, [P, I, I, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_sssp_worklist": (I, [P, I, I, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_sssp_worklist_noalloc": (I, [P, I, I, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_verify_distances": (I, [P, I, I, P, P, P, P, C.POINTER(C.c_longlong)]),
         "gdb_mst": (I, [P, I, I, P, P, P, C.POINTER(C.c_ulonglong), C.POINTER(I), C.POINTER(I),
                         C.POINTER(Opts), C.POINTER(Stats)]),

@@ -26,8 +26,41 @@
 //                      wave.
 //   relaunch         - one launch per iteration with a host loop reading the
 //                      output size (the non-persistent schedule).
+#include <mutex>
 #include "graph_common.hpp"
 #include "runtime.hpp"
+
+// Scratch of gdb_sssp_worklist_noalloc: the stamp array and both queues
+// (3 x n ints) plus flags and work counters, for n up to 2^22 vertices.
+// Each device that loads the library holds its own copy until the process
+// ends.  Host code never passes `scratch` itself to a kernel: its device
+// address comes from hipGetSymbolAddress (resolve_noalloc_scratch).
+#define GDB_NOALLOC_SCRATCH_BYTES ((size_t)3 * (4u << 22) + 4096)
+static __device__ __align__(256) unsigned char scratch[GDB_NOALLOC_SCRATCH_BYTES];
+
+namespace gdbrt {
+
+int resolve_noalloc_scratch(gdb_ctx* ctx) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return GDB_E_INVALID;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[ctx->device]) {
+    int cur = -1;
+    GDB_HIP(hipGetDevice(&cur));
+    if (cur != ctx->device) GDB_HIP(hipSetDevice(ctx->device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != ctx->device) GDB_HIP(hipSetDevice(cur));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    addr[ctx->device] = p;
+  }
+  ctx->noalloc_scratch = addr[ctx->device];
+  return GDB_SUCCESS;
+}
+
+}  // namespace gdbrt
 
 namespace {
 
@@ -321,9 +354,12 @@ __global__ void k_verify(const int* __restrict__ row, const int* __restrict__ co
   if (bad) atomicAdd(nerr, bad);
 }
 
+// `fixed_ws`: a caller-provided device buffer of `fixed_bytes` bytes that
+// replaces the context's grow-only workspace (no allocation), or nullptr.
 template <bool WEIGHTED>
 int run_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* w, int source,
-                 unsigned* dist, const gdb_opts* opts, gdb_stats* stats) {
+                 unsigned* dist, const gdb_opts* opts, gdb_stats* stats, void* fixed_ws = nullptr,
+                 size_t fixed_bytes = 0) {
   using namespace gdbrt;
   if (!ctx || n < 0 || m < 0 || (n > 0 && (!row || !dist)) || (m > 0 && !col) ||
       (WEIGHTED && m > 0 && !w) || (n > 0 && (source < 0 || source >= n)))
@@ -337,9 +373,16 @@ int run_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col, con
     return GDB_SUCCESS;
   }
   const size_t bytes = carve_bytes<int>(n) * 3 + carve_bytes<int>(32) + carve_bytes<long long>(2);
-  int rc = ensure_workspace(ctx, bytes);
-  if (rc) return rc;
-  Carve cv(ctx->ws);
+  void* base = fixed_ws;
+  int rc = GDB_SUCCESS;
+  if (base) {
+    if (bytes > fixed_bytes) return GDB_E_SCRATCH;
+  } else {
+    rc = ensure_workspace(ctx, bytes);
+    if (rc) return rc;
+    base = ctx->ws;
+  }
+  Carve cv(base);
   int* stamp = cv.take<int>(n);
   int* q0 = cv.take<int>(n);
   int* q1 = cv.take<int>(n);
@@ -432,6 +475,16 @@ extern "C" int gdb_sssp_worklist(gdb_ctx* ctx, int n, int m, const int* row, con
                                  const int* w, int source, unsigned* dist, const gdb_opts* opts,
                                  gdb_stats* stats) {
   return run_worklist<true>(ctx, n, m, row, col, w, source, dist, opts, stats);
+}
+
+extern "C" int gdb_sssp_worklist_noalloc(gdb_ctx* ctx, int n, int m, const int* row, const int* col,
+                                         const int* w, int source, unsigned* dist, const gdb_opts* opts,
+                                         gdb_stats* stats) {
+  if (!ctx) return GDB_E_INVALID;
+  if (!ctx->noalloc_scratch) return GDB_E_INVALID;  // resolved by gdb_ctx_create
+  if (n > GDB_NOALLOC_MAX_N) return GDB_E_SCRATCH;
+  return run_worklist<true>(ctx, n, m, row, col, w, source, dist, opts, stats, ctx->noalloc_scratch,
+                            GDB_NOALLOC_SCRATCH_BYTES);
 }
 
 extern "C" int gdb_verify_distances(gdb_ctx* ctx, int n, int m, const int* row, const int* col,

@@ -290,7 +290,9 @@ int gdb_ctx_create(gdb_ctx** out) {
   GDB_HIP(hipHostMalloc((void**)&ctx->h_flag, sizeof(int) * 4, hipHostMallocDefault));
   GDB_HIP(hipEventCreate(&ctx->ev0));
   GDB_HIP(hipEventCreate(&ctx->ev1));
-  int rc = init_ctx(ctx, 0);
+  int rc = resolve_noalloc_scratch(ctx);
+  if (rc) return rc;
+  rc = init_ctx(ctx, 0);
   if (rc) return rc;
   *out = ctx;
   return GDB_SUCCESS;

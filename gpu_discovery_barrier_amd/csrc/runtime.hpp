@@ -24,6 +24,9 @@ struct gdb_ctx {
   // timed region; never freed while the context lives).
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  // Device address, on this context's device, of the library's static
+  // scratch array (gdb_sssp_worklist_noalloc); resolved at context creation.
+  void* noalloc_scratch = nullptr;
   // Pinned host word for stop-flag readbacks of the relaunch schedule.
   int* h_flag = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -80,6 +83,11 @@ inline int default_wgs(const gdb_opts* o) { return (o && o->wgs > 0) ? o->wgs : 
 
 // Reads the participating count of the last launch (blocking).
 int read_participating(gdb_ctx* ctx, int* out);
+
+// Resolves, once per device, the device address of the static scratch array
+// of the allocation-free worklist entry point and stores it in the context
+// (defined beside that array, in apps_worklist.hip).
+int resolve_noalloc_scratch(gdb_ctx* ctx);
 
 // Start / stop the solve timer on the context stream.
 int timer_start(gdb_ctx* ctx);

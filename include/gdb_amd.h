@@ -60,6 +60,8 @@ extern "C" {
 #define GDB_E_UNSAFE (-4)    /* skip-mode launch larger than the residency bound */
 #define GDB_E_OVERFLOW (-5)  /* a device queue overflowed                      */
 #define GDB_E_NOMEM (-6)     /* device allocation failed                       */
+#define GDB_E_SCRATCH (-7)   /* input larger than the library's static scratch
+                                (gdb_sssp_worklist_noalloc: n > GDB_NOALLOC_MAX_N) */
 
 /* ---- protocol / barrier / schedule selectors ---------------------------- */
 #define GDB_PROTOCOL_LOCKFREE 0 /* single atomic ticket word (default)        */
@@ -205,6 +207,20 @@ int gdb_bfs_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col,
 /* Lonestar worklist SSSP (label-correcting, atomic min). */
 int gdb_sssp_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* w,
                       int source, unsigned* dist, const gdb_opts* opts, gdb_stats* stats);
+/* gdb_sssp_worklist without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot call hipMalloc (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * Same arguments, schedules, results and stats as gdb_sssp_worklist; its
+ * queues and counters live in a file-scope device array of the library
+ * (48 MiB + 4 KiB per GPU, held until the process ends) instead of the
+ * context's workspace, so n is limited to GDB_NOALLOC_MAX_N (larger inputs
+ * return GDB_E_SCRATCH).  Every call on one GPU shares that GPU's array and
+ * re-initialises what it reads; calls that overlap on one GPU (from
+ * several threads or contexts) race and must be serialised by the caller. */
+#define GDB_NOALLOC_MAX_N (1 << 22)
+int gdb_sssp_worklist_noalloc(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* w,
+                              int source, unsigned* dist, const gdb_opts* opts, gdb_stats* stats);
+
 /* dverifysolution: counts edges (u,v,w) with dist[u] + w < dist[v]; w = 1
  * when `w` is NULL (BFS). */
 int gdb_verify_distances(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const int* w,
