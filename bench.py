@@ -29,9 +29,25 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# wgs, groups, near/far band, local hops, selector mode (0 none, 4 MLP)
-TUNED_C2 = (128, 256, 4096, 256, 0)
 GRID_SIDE = 2048
+
+
+def tuned_c2():
+    """The batched engine's shape for the grid from the library's tuning table
+    (gpu_discovery_barrier_amd/tune_table.txt, written by the autotuner):
+    ((wgs, groups, near/far band, local hops, selector mode), team groups,
+    LDS queue tile)."""
+    from gpu_discovery_barrier_amd import autotune
+
+    r = autotune.lookup("batch", GRID_SIDE * GRID_SIDE)
+    if r is None:
+        return (128, 256, 4096, 256, 0), 128, 2048
+    return ((r["wgs"], r["groups"], r["delta"], r["hops"], 0), r["team_groups"] or max(1, r["groups"] // 2),
+            r["lds_queue"] or 2048)
+
+
+# wgs, groups, near/far band, local hops, selector mode (0 none, 4 MLP)
+TUNED_C2, TUNED_C2_TEAM, TUNED_C2_QCAP = tuned_c2()
 
 
 def log(*a):
@@ -58,22 +74,43 @@ def traffic_from_profiles(shape: tuple):
         return None
 
 
-def cpu_baseline(g, seconds_cap: float = 30.0) -> dict:
+def cpu_baseline(g, want_b=None, want_s=None) -> dict:
+    """The worklist BFS + SSSP on the host's cores: persistent threads (one per
+    core, at most 16, the GPU box's CPU share) running the same iterations
+    with a CPU XF barrier; one warm-up, then the median of 5 solves, timed
+    around the solve only.  The single-thread sequential restatement is timed
+    beside it."""
     import oracle
 
-    t0 = time.perf_counter()
-    dist_b, _ = oracle.bfs_worklist_ref(g.row, g.col, 0)
-    t1 = time.perf_counter()
-    dist_s, _ = oracle.sssp_worklist_ref(g.row, g.col, g.w, 0)
-    t2 = time.perf_counter()
+    threads = min(16, os.cpu_count() or 1)
+    tb, ts = [], []
+    for rep in range(6):
+        t0 = time.perf_counter()
+        dist_b, _ = oracle.worklist_mt(g.row, g.col, None, 0, threads=threads)
+        t1 = time.perf_counter()
+        dist_s, _ = oracle.worklist_mt(g.row, g.col, g.w, 0, threads=threads)
+        t2 = time.perf_counter()
+        if rep > 0:
+            tb.append(t1 - t0)
+            ts.append(t2 - t1)
+    exact = (want_b is None or bool((dist_b == want_b).all())) and (want_s is None or bool((dist_s == want_s).all()))
     mr = reached_edges(g, dist_b, oracle.LONESTAR_INF) + reached_edges(g, dist_s, oracle.LONESTAR_INF)
+    t_mt = float(np.median(tb)) + float(np.median(ts))
+    t0 = time.perf_counter()
+    oracle.bfs_worklist_ref(g.row, g.col, 0)
+    t1 = time.perf_counter()
+    oracle.sssp_worklist_ref(g.row, g.col, g.w, 0)
+    t2 = time.perf_counter()
     return {
-        "value": mr / (t2 - t0) / 1e9,
+        "value": round(mr / t_mt / 1e9, 6),
         "unit": "GTEPS",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"one BFS + one SSSP worklist solve of the full {GRID_SIDE}x{GRID_SIDE} grid "
-                  f"(oracle restatement, single thread): bfs {t1 - t0:.2f}s, sssp {t2 - t1:.2f}s",
+        "sample": f"one worklist BFS + one worklist SSSP of the full {GRID_SIDE}x{GRID_SIDE} grid on {threads} "
+                  f"persistent threads with a CPU XF barrier (oracle/oracle.c orc_worklist_mt), median of 5 after "
+                  f"one warm-up: bfs {np.median(tb):.3f}s, sssp {np.median(ts):.3f}s; outputs equal the GPU's: {exact}",
+        "single_thread": {"value": round(mr / (t2 - t0) / 1e9, 6), "bfs_s": round(t1 - t0, 3),
+                          "sssp_s": round(t2 - t1, 3), "kind": "sequential restatement"},
     }
 
 
@@ -357,6 +394,53 @@ def bench_c4(args) -> int:
     return 0 if exact else 3
 
 
+def bench_probe(args) -> int:
+    """Memory-latency probes and protocol / barrier costs on this device:
+    dependent-access chains (plain load, agent-scope load, returning atomic)
+    over buffers served by L2, the Infinity Cache and HBM, in coarse- and
+    fine-grained memory; dependent atomic-min steps (32- vs 64-bit) from one
+    CU and from 256 CUs; the discovery protocol's cost per lock kind for 1000
+    launched groups; every barrier kind at 8..1024 groups for wgs 128 / 256."""
+    import gpu_discovery_barrier_amd as gdb
+
+    ctx = gdb.Context()
+    steps = 20000
+    chain = {}
+    for nbytes, label in ((1 << 20, "1MiB"), (16 << 20, "16MiB"), (128 << 20, "128MiB"), (1 << 30, "1GiB")):
+        for kind, kn in ((0, "load"), (2, "agent_load"), (1, "atomic")):
+            chain[f"{kn}_{label}"] = round(ctx.probe_chain(kind, nbytes, False, steps), 1)
+        log(f"chain {label}: {[chain[k] for k in chain if k.endswith(label)]}")
+    for kind, kn in ((0, "load"), (2, "agent_load"), (1, "atomic")):
+        chain[f"{kn}_1MiB_fine"] = round(ctx.probe_chain(kind, 1 << 20, True, steps), 1)
+    atom = {}
+    for bits in (32, 64):
+        for groups, wgs in ((1, 1024), (256, 1024), (256, 128)):
+            atom[f"{bits}bit_{groups}x{wgs}"] = round(ctx.probe_atomics(bits, groups, wgs, 2000), 1)
+    ping = {}
+    for mode, mn in ((0, "relaxed"), (1, "release_acquire"), (2, "fetch_add_acq_rel")):
+        for fine in (False, True):
+            for same in (True, False):
+                ns, xc = ctx.probe_pingpong(mode, fine, same, 5000)
+                ping[f"{mn}_{'fine' if fine else 'coarse'}_{'same_xcd' if same else 'two_xcds'}"] = {
+                    "ns": round(ns, 1), "xcds": xc}
+    log(f"pingpong: {ping}")
+    prot = {}
+    for lock, name in ((gdb.PROTOCOL_LOCKFREE, "lock-free"), (gdb.PROTOCOL_TICKET, "ticket"),
+                       (gdb.PROTOCOL_SPIN, "spin")):
+        ctx.configure(lock, gdb.BARRIER_AUTO)
+        for wgs in (64, 256):
+            ns = ctx.time_protocol(1000, wgs)
+            prot[f"{name}_wgs{wgs}"] = {"ms": round(ns / 1e6, 4), "P": ctx.participating()}
+    ctx.configure(gdb.PROTOCOL_LOCKFREE, gdb.BARRIER_AUTO)
+    bars = {str(w): barrier_table(ctx, w, [8, 32, 128, 256, 1024]) for w in (128, 256)}
+    out = {"metric": "memory latency and synchronisation costs on 1 MI355X", "unit": "ns / us",
+           "chain_ns_per_access": chain, "atomic_ns_per_dependent_step": atom, "pingpong_ns_per_handoff": ping,
+           "protocol_1000_groups": prot, "barrier_us": bars, "steps": steps, "data": "synthetic"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def copy_calibration(args) -> int:
     """Three device-to-device copies of 256 MiB (the runtime's copy kernel):
     the reference reading for scaling FETCH_SIZE / WRITE_SIZE counters."""
@@ -384,16 +468,18 @@ def main() -> int:
                     help="c2: wgs,groups,delta,hops,selector (skips the autotuner; selector 0 none, 4 MLP)")
     ap.add_argument("--engine-only", action="store_true",
                     help="c2 / c3: time the engine's kernels only (profiler runs)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "copy"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "copy", "probe"],
                     help="c2 (default): the headline grid BFS + SSSP; c1 colour R-MAT 14; "
-                         "c3 MIS + BC R-MAT 22; c4 4096-graph batch; copy: counter calibration")
+                         "c3 MIS + BC R-MAT 22; c4 4096-graph batch; copy: counter calibration; "
+                         "probe: memory latency, protocol and barrier costs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(this build runs a single MI355X)", file=sys.stderr)
         return 2
     if args.config != "c2":
-        return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "copy": copy_calibration}[args.config](args)
+        return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "copy": copy_calibration,
+                "probe": bench_probe}[args.config](args)
 
     import gpu_discovery_barrier_amd as gdb
     from gpu_discovery_barrier_amd import autotune
@@ -428,42 +514,41 @@ def main() -> int:
             return None
         return gdb.make_selector(mode, narrow_groups=narrow, weights=weights)
 
-    def solve_batched(wgs, groups, delta, hops=0, mode=gdb.SEL_NONE):
+    team, qcap = TUNED_C2_TEAM, TUNED_C2_QCAP
+
+    def solve_batched(wgs, groups, delta, hops=0, mode=gdb.SEL_NONE, team_groups=None, lds_queue=None):
         # One launch, two teams: BFS and near/far SSSP side by side.  The
         # grid is symmetric, so it is its own in-edge CSR for pull arms.
         tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0, hops=hops, transpose=dg),
                  gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta, hops=hops, transpose=dg)]
-        rounds, st = gdb.batch_solve(ctx, tasks, max(1, groups // 2),
-                                     gdb.Opts.make(wgs=wgs, groups=groups), selector=selector(mode))
+        rounds, st = gdb.batch_solve(ctx, tasks, team_groups or team,
+                                     gdb.Opts.make(wgs=wgs, groups=groups, lds_queue=lds_queue or qcap),
+                                     selector=selector(mode))
         last_tasks[:] = tasks
         return rounds, st
 
-    # Autotuning of the batched engine (untimed): launch shape, near/far
-    # band, local hops, and whether the learned selector schedules the
-    # iterations (it needs hops = 0).
-    modes = [gdb.SEL_NONE] + ([gdb.SEL_MLP] if weights is not None else [])
+    # The shape: the tuning table's row for this workload (written by
+    # python -m gpu_discovery_barrier_amd.autotune, which sweeps wgs, groups,
+    # team, band, local hops and the LDS queue tile), or --shape, or --tune
+    # (reruns that sweep here, plus the learned selector, which needs hops 0).
     if args.shape:
         wgs, groups, delta, hops, mode = (int(x) for x in args.shape.split(","))
+        team = max(1, groups // 2)
     elif not args.tune:
-        # The autotuner's pick on this workload (python bench.py --tune):
-        # 180 shapes; the best few within 2-4% of each other.
         wgs, groups, delta, hops, mode = TUNED_C2
     else:
-        trials = []
-        for w in (128, 256, 512):
-            for gr in (64, 128, 256, 512):
-                gr = min(gr, gdb.max_resident_groups(w, 0))
-                for dl in (2048, 4096, 8192):
-                    for hp in (0, 64, 256, 1024):
-                        for md in modes:
-                            if md != gdb.SEL_NONE and hp > 0:
-                                continue
-                            _, st = solve_batched(w, gr, dl, hp, md)
-                            trials.append((st.kernel_ms, w, gr, dl, hp, md))
-        trials.sort()
-        _, wgs, groups, delta, hops, mode = trials[0]
-        log(f"autotune: wgs={wgs} groups={groups} delta={delta} hops={hops} selector={mode} "
-            f"({trials[0][0]:.2f} ms); {len(trials)} shapes tried")
+        from gpu_discovery_barrier_amd import autotune
+
+        r, trials = autotune.tune_batch_grid(ctx)
+        wgs, groups, team, delta, hops, qcap = r["wgs"], r["groups"], r["team_groups"], r["delta"], r["hops"], \
+            r["lds_queue"]
+        mode = gdb.SEL_NONE
+        if weights is not None:
+            ms_mlp = min(solve_batched(wgs, groups, delta, 0, gdb.SEL_MLP)[1].kernel_ms for _ in range(2))
+            if ms_mlp < min(t[0] for t in trials):
+                hops, mode = 0, gdb.SEL_MLP
+        log(f"autotune: wgs={wgs} groups={groups} team={team} delta={delta} hops={hops} lds_queue={qcap} "
+            f"selector={mode}; {len(trials)} shapes tried")
 
     for _ in range(args.warmup):
         solve_batched(wgs, groups, delta, hops, mode)
@@ -502,7 +587,7 @@ def main() -> int:
     # Barrier vs relaunch: every barrier kind at the team shape (one team =
     # groups / 2) and at 8, 32, 256 and 1024 groups; relaunch (i) back to back,
     # (ii) with the reference host loop's blocking readback.
-    P = max(1, groups // 2)
+    P = team
     bars = barrier_table(ctx, wgs, sorted({8, 32, P, 256, 1024}))
     us_bar = bars["counter" if P < 64 else "tree"][str(P)]["us_per_barrier"]
     us_relaunch = bars["relaunch"][str(P)]
@@ -541,8 +626,9 @@ def main() -> int:
                         "5% edges removed (seed 2), weights U[1,255], source 0",
             "n": g.n, "m": g.m, "m_reached": mr,
             "engine": "batched (BFS and near/far SSSP teams in one persistent launch)",
-            "wgs": wgs, "groups": groups, "team_groups": max(1, groups // 2), "delta": delta,
-            "local_hops": hops,
+            "wgs": wgs, "groups": groups, "team_groups": team, "delta": delta,
+            "local_hops": hops, "lds_queue": qcap, "shape_source": "tuning table" if not (args.shape or args.tune)
+            else ("--shape" if args.shape else "--tune"),
             "selector": {gdb.SEL_NONE: "none (push on the whole team)", gdb.SEL_MLP: "learned MLP"}[mode],
             "narrow_groups": narrow,
             "arm_iterations": {"bfs": list(last_tasks[0].arm_iters), "sssp": list(last_tasks[1].arm_iters)},
@@ -576,7 +662,7 @@ def main() -> int:
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(g)
+        out["cpu_baseline"] = cpu_baseline(g, db, ds)
     ctx.close()
     print(json.dumps(out), flush=True)
     return 0 if bit_exact and nerr == 0 else 3

@@ -48,12 +48,29 @@ class Opts(C.Structure):
         ("variant", C.c_int),
         ("barrier", C.c_int),
         ("relaunch_check", C.c_int),
+        ("lds_queue", C.c_int),
     ]
 
     @classmethod
     def make(cls, wgs=256, groups=0, schedule=SCHED_PERSISTENT, variant=VARIANT_PLAIN,
-             barrier=-1, relaunch_check=1) -> "Opts":
-        return cls(wgs, groups, schedule, variant, barrier, relaunch_check)
+             barrier=-1, relaunch_check=1, lds_queue=0) -> "Opts":
+        """wgs = 0 takes the launch shape from the tuning table (tuned
+        persistent variant, batched engine, MST)."""
+        return cls(wgs, groups, schedule, variant, barrier, relaunch_check, lds_queue)
+
+
+class TuneEntry(C.Structure):
+    """gdb_tune_entry (include/gdb_amd.h): one row of the tuning table."""
+    _fields_ = [
+        ("solver", C.c_char * 16),
+        ("n_max", C.c_longlong),
+        ("wgs", C.c_int),
+        ("groups", C.c_int),
+        ("team_groups", C.c_int),
+        ("delta", C.c_int),
+        ("hops", C.c_int),
+        ("lds_queue", C.c_int),
+    ]
 
 
 class Stats(C.Structure):
@@ -165,6 +182,11 @@ def _declare(lib) -> None:
         "gdb_barrier_bench": (I, [P, I, I, I, C.POINTER(C.c_double), C.POINTER(I)]),
         "gdb_barrier_check": (I, [P, I, I, I, C.POINTER(C.c_longlong), C.POINTER(I)]),
         "gdb_relaunch_bench": (I, [P, I, I, I, I, C.POINTER(C.c_double)]),
+        "gdb_tune_load": (I, [C.c_char_p]),
+        "gdb_tune_lookup": (I, [C.c_char_p, C.c_longlong, C.POINTER(TuneEntry)]),
+        "gdb_probe_chain": (I, [P, I, C.c_size_t, I, I, C.POINTER(C.c_double)]),
+        "gdb_probe_atomics": (I, [P, I, I, I, I, C.POINTER(C.c_double)]),
+        "gdb_probe_pingpong": (I, [P, I, I, I, I, C.POINTER(C.c_double), C.POINTER(I)]),
         "gdb_color": (I, [P, I, I, P, P, P, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_csr_free": (None, [C.POINTER(HostCSRStruct)]),
         "gdb_gen_rmat": (I, [I, I, C.c_double, C.c_double, C.c_double, C.c_uint64, I, I,
@@ -447,10 +469,46 @@ class Context:
               "gdb_barrier_check")
         return bad.value, p.value
 
+    def probe_chain(self, kind: int, nbytes: int, fine: bool, steps: int) -> float:
+        """ns per dependent access (kind 0 load, 1 atomic add, 2 agent-scope load)."""
+        ns = C.c_double(0)
+        check(lib().gdb_probe_chain(self.h, kind, nbytes, int(fine), steps, C.byref(ns)), "gdb_probe_chain")
+        return ns.value
+
+    def probe_atomics(self, bits: int, groups: int, wgs: int, iters: int) -> float:
+        """ns per step of dependent returning atomic-min operations (every lane)."""
+        ns = C.c_double(0)
+        check(lib().gdb_probe_atomics(self.h, bits, groups, wgs, iters, C.byref(ns)), "gdb_probe_atomics")
+        return ns.value
+
+    def probe_pingpong(self, mode: int, fine: bool, same_xcd: bool, iters: int) -> tuple[float, list]:
+        """(ns per handoff between two groups, their XCD ids)."""
+        ns, x = C.c_double(0), (C.c_int * 2)()
+        check(lib().gdb_probe_pingpong(self.h, mode, int(fine), int(same_xcd), iters, C.byref(ns), x),
+              "gdb_probe_pingpong")
+        return ns.value, [x[0], x[1]]
+
     def relaunch_bench(self, wgs: int, groups: int, iters: int, mode: int = 0) -> float:
         ms = C.c_double(0)
         check(lib().gdb_relaunch_bench(self.h, wgs, groups, iters, mode, C.byref(ms)), "gdb_relaunch_bench")
         return ms.value
+
+
+def tune_load(path: str | None = None) -> int:
+    """Replaces the library's tuning table with a file's rows (None: the
+    default file); returns the number of rows."""
+    rc = lib().gdb_tune_load(None if path is None else str(path).encode())
+    if rc < 0:
+        raise GdbError(rc, "gdb_tune_load")
+    return rc
+
+
+def tune_lookup(solver: str, n: int) -> dict | None:
+    """The tuning-table row a solver of graph size n uses, or None."""
+    e = TuneEntry()
+    if lib().gdb_tune_lookup(solver.encode(), int(n), C.byref(e)) != GDB_SUCCESS:
+        return None
+    return {k: (getattr(e, k).decode() if k == "solver" else getattr(e, k)) for k, _ in TuneEntry._fields_}
 
 
 def max_resident_groups(wgs: int, lms: int = 0) -> int:

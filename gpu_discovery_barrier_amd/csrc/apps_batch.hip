@@ -92,7 +92,9 @@ __device__ __forceinline__ bool mlp_reuse(const IterState& S, int nin) {
          (long long)nin * 4 <= (long long)S.eval_nin * 5 && (long long)nin * 5 >= (long long)S.eval_nin * 4;
 }
 
-constexpr int kLocalQ = 2048;  // LDS queue capacity per buffer (ints)
+// LDS queue capacity per buffer (ints): gdb_opts.lds_queue, in dynamic LDS.
+constexpr int kLocalQDefault = 2048;
+constexpr int kLocalQMin = 256, kLocalQMax = 4096;
 constexpr int kMaxHops = 4096;  // local-step budget limit per team iteration
 
 struct Team {
@@ -101,7 +103,8 @@ struct Team {
   int* ts;        // this team's state lines
   int* gen_cache; // LDS
   int G, rank;
-  int (*lq)[kLocalQ];  // LDS: two local queue buffers
+  int* lq;             // LDS: two local queue buffers of qcap ints each
+  int qcap;
   int* lcnt;           // LDS: three rotating local queue counters
   __device__ int word_off(int w) const { return w * kLine; }
   __device__ int* word(int w) const { return ts + w * kLine; }
@@ -375,7 +378,8 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
   // dedupe team-queue pushes with gstamp instead, because a vertex pushed
   // to an LDS queue may be relaxed again before the iteration ends.
   __shared__ int lds_n;
-  int (*lq)[kLocalQ] = T.lq;
+  int* const lq = T.lq;
+  const int qcap = T.qcap;
   int* lcnt = T.lcnt;
   const int n = k.n;
   const int G = T.G;
@@ -665,7 +669,7 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
           for (int j = 0; j < kChunk; ++j) {
             if (cls[j] == 3) {
               const int at = pos + __popcll(m3[j] & lt);
-              if (at < kLocalQ) lq[lb][at] = dst[j];
+              if (at < qcap) lq[lb * qcap + at] = dst[j];
               else to_team[j] = atomicExch(&gstamp[dst[j]], it) != it;
             }
             pos += __popcll(m3[j]);
@@ -733,20 +737,20 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
       int s = 1;
       for (; s <= hops; ++s) {
         __syncthreads();
-        const int ln = min(lcnt[s % 3], kLocalQ);
+        const int ln = min(lcnt[s % 3], qcap);
         if (ln == 0) break;
         if (threadIdx.x == 0) lcnt[(s + 2) % 3] = 0;
-        process(lq[s & 1], ln, (int)threadIdx.x, (int)blockDim.x, (s + 1) & 1, &lcnt[(s + 1) % 3]);
+        process(lq + (s & 1) * qcap, ln, (int)threadIdx.x, (int)blockDim.x, (s + 1) & 1, &lcnt[(s + 1) % 3]);
       }
       if (hops > 0 && s > hops) {
         // Budget spent: flush the last local buffer to the team queue.
         __syncthreads();
-        const int ln = min(lcnt[s % 3], kLocalQ);
+        const int ln = min(lcnt[s % 3], qcap);
         GDB_FOR_UNIFORM(i, valid, (int)threadIdx.x, (int)blockDim.x, ln) {
           int v = 0;
           bool p = false;
           if (valid) {
-            v = lq[s & 1][i];
+            v = lq[(s & 1) * qcap + i];
             p = atomicExch(&gstamp[v], it) != it;
           }
           gdb::wave_push(p, v, outq, out_tail);
@@ -989,12 +993,15 @@ template <int WGS, bool SEL>
 __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(WGS == 512 && !SEL ? 6 : 1))) void k_batch(gdb_kernel_ctx_t* ctx, const DevTask* __restrict__ tasks,
                                                int ntasks, int G, int* team_state, int* next_task,
                                                int* results, int* teams_used,
-                                               unsigned long long* work, gdbsel::Cfg sel) {
+                                               unsigned long long* work, gdbsel::Cfg sel, int qcap) {
   DISCOVERY_PROTOCOL(ctx);
   __shared__ int gen_cache;
-  __shared__ int lq[2][kLocalQ];
   __shared__ int lcnt[3];
-  extern __shared__ unsigned short sel_w[];  // MLP weights (dynamic LDS, MLP mode only)
+  // Dynamic LDS: the two local queue buffers (qcap ints each), then the MLP
+  // weights (MLP mode only).
+  extern __shared__ int dyn_lds[];
+  int* const lq = dyn_lds;
+  unsigned short* const sel_w = (unsigned short*)(dyn_lds + 2 * qcap);
   const int P = p_get_num_groups(ctx, &local_ctx);
   const int id = p_get_group_id(ctx, &local_ctx);
   if (G > P) G = P;
@@ -1006,7 +1013,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(WGS == 512 
   const bool mlp = SEL && sel.mode == GDB_SEL_MLP && sel.w;
   if (mlp)
     for (int i = threadIdx.x; i < gdbsel::kParams; i += blockDim.x) sel_w[i] = sel.w[i];
-  Team T{ctx, &local_ctx, team_state + (size_t)team * TW_WORDS * kLine, &gen_cache, G, id % G, lq, lcnt};
+  Team T{ctx, &local_ctx, team_state + (size_t)team * TW_WORDS * kLine, &gen_cache, G, id % G, lq, qcap, lcnt};
   if (id == 0 && threadIdx.x == 0) *teams_used = nteams;
   int Gn = sel.narrow_groups > 0 ? sel.narrow_groups : 1;
   if (Gn > G) Gn = G;
@@ -1058,7 +1065,7 @@ namespace {
 // workspace, or an allocation of its own for a captured graph).
 struct BatchPlan {
   gdb_ctx* ctx = nullptr;
-  int ntasks = 0, G = 1, groups = 0, wgs = 256;
+  int ntasks = 0, G = 1, groups = 0, wgs = 256, qcap = kLocalQDefault;
   size_t dyn_lds = 0;
   gdbsel::Cfg cfg{GDB_SEL_NONE, 0, 1, 0u, nullptr, nullptr, 0, nullptr};
   std::vector<int> order;  // order[i] = caller's index of table entry i
@@ -1075,10 +1082,17 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
   if (!ctx || ntasks <= 0 || !tasks || team_groups < 0) return GDB_E_INVALID;
   gdb_opts o{};
   if (opts) o = *opts;
+  {
+    long long nmax = 0;
+    for (int t = 0; t < ntasks; ++t) nmax = std::max<long long>(nmax, tasks[t].n);
+    apply_tuning("batch", nmax, &o, &team_groups);
+  }
   P->ctx = ctx;
   P->ntasks = ntasks;
   P->wgs = default_wgs(&o);
   P->G = team_groups > 0 ? team_groups : 1;
+  P->qcap = o.lds_queue > 0 ? o.lds_queue : kLocalQDefault;
+  if (P->qcap < kLocalQMin || P->qcap > kLocalQMax) return GDB_E_INVALID;
   if (sel) {
     if (sel->mode < GDB_SEL_NONE || sel->mode > GDB_SEL_MLP || sel->narrow_groups < 0 ||
         sel->trace_capacity < 0 || (sel->trace && !sel->trace_count))
@@ -1116,7 +1130,8 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
                     nullptr, a.trow, a.tcol, a.tw};
     scratch_total += (ints + 63) & ~(size_t)63;
   }
-  P->dyn_lds = (cfg.mode == GDB_SEL_MLP && cfg.w) ? sizeof(unsigned short) * gdbsel::kParams : 0;
+  P->dyn_lds = sizeof(int) * 2 * (size_t)P->qcap +
+               ((cfg.mode == GDB_SEL_MLP && cfg.w) ? sizeof(unsigned short) * gdbsel::kParams : 0);
   int rc = gdb_dispatch_wgs(P->wgs, [&](auto W) -> int {
     constexpr int WG = decltype(W)::value;
     const void* kern = cfg.mode != GDB_SEL_NONE ? (const void*)k_batch<WG, true> : (const void*)k_batch<WG, false>;
@@ -1173,10 +1188,12 @@ int enqueue_kernel(const BatchPlan& P) {
     constexpr int WG = decltype(W)::value;
     if (P.cfg.mode != GDB_SEL_NONE)
       hipLaunchKernelGGL((k_batch<WG, true>), dim3(P.groups), dim3(WG), P.dyn_lds, ctx->stream, ctx->d,
-                         P.d_tasks, P.ntasks, P.G, P.team_state, next_task, results, teams_used, P.work, P.cfg);
+                         P.d_tasks, P.ntasks, P.G, P.team_state, next_task, results, teams_used, P.work, P.cfg,
+                         P.qcap);
     else
-      hipLaunchKernelGGL((k_batch<WG, false>), dim3(P.groups), dim3(WG), 0, ctx->stream, ctx->d, P.d_tasks,
-                         P.ntasks, P.G, P.team_state, next_task, results, teams_used, P.work, P.cfg);
+      hipLaunchKernelGGL((k_batch<WG, false>), dim3(P.groups), dim3(WG), P.dyn_lds, ctx->stream, ctx->d,
+                         P.d_tasks, P.ntasks, P.G, P.team_state, next_task, results, teams_used, P.work, P.cfg,
+                         P.qcap);
     GDB_HIP(hipGetLastError());
     return GDB_SUCCESS;
   });

@@ -707,6 +707,7 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
     return GDB_E_INVALID;
   gdb_opts o{};
   if (opts) o = *opts;
+  if (o.variant == GDB_VARIANT_TUNED && o.schedule == GDB_SCHED_PERSISTENT) gdbrt::apply_tuning("bc", n, &o);
   const int wgs = default_wgs(&o);
   gdb_stats st{};
   if (n == 0) {

@@ -351,6 +351,7 @@ extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* 
     return GDB_E_INVALID;
   gdb_opts o{};
   if (opts) o = *opts;
+  if (o.variant == GDB_VARIANT_TUNED && o.schedule == GDB_SCHED_PERSISTENT) gdbrt::apply_tuning("color", n, &o);
   const int wgs = default_wgs(&o);
   gdb_stats s{};
   if (n == 0) {

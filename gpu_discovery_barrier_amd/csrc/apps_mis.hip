@@ -259,6 +259,7 @@ extern "C" int gdb_mis(gdb_ctx* ctx, int n, int m, const int* row, const int* co
     return GDB_E_INVALID;
   gdb_opts o{};
   if (opts) o = *opts;
+  if (o.variant == GDB_VARIANT_TUNED && o.schedule == GDB_SCHED_PERSISTENT) gdbrt::apply_tuning("mis", n, &o);
   const int wgs = default_wgs(&o);
   gdb_stats st{};
   if (n == 0) {

@@ -175,6 +175,7 @@ extern "C" int gdb_mst(gdb_ctx* ctx, int n, int m, const int* row, const int* co
   if (!ctx || n < 0 || m < 0 || (n > 0 && !row) || (m > 0 && (!col || !w))) return GDB_E_INVALID;
   gdb_opts o{};
   if (opts) o = *opts;
+  gdbrt::apply_tuning("mst", n, &o);
   const int wgs = default_wgs(&o);
   gdb_stats st{};
   if (n == 0) {

@@ -165,6 +165,7 @@ extern "C" int gdb_sssp_pull(gdb_ctx* ctx, int n, int m, const int* row_t, const
     return GDB_E_INVALID;
   gdb_opts o{};
   if (opts) o = *opts;
+  if (o.variant == GDB_VARIANT_TUNED && o.schedule == GDB_SCHED_PERSISTENT) gdbrt::apply_tuning("sssp_pull", n, &o);
   const int wgs = default_wgs(&o);
   gdb_stats st{};
   if (n == 0) {

@@ -366,6 +366,7 @@ int run_worklist(gdb_ctx* ctx, int n, int m, const int* row, const int* col, con
     return GDB_E_INVALID;
   gdb_opts o{};
   if (opts) o = *opts;
+  if (o.variant == GDB_VARIANT_TUNED && o.schedule == GDB_SCHED_PERSISTENT) gdbrt::apply_tuning(WEIGHTED ? "sssp_wl" : "bfs_wl", n, &o);
   const int wgs = default_wgs(&o);
   gdb_stats st{};
   if (n == 0) {

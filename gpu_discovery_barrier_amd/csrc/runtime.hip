@@ -3,10 +3,13 @@
 //
 // Host API counterpart of code/discovery_protocol/api/discovery.h and of the
 // occupancy harness in code/experiments/occupancy_tests/src/.
+#include <dlfcn.h>
 #include <chrono>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <vector>
 #include "gdb_discovery.hpp"
 #include "runtime.hpp"
 
@@ -197,6 +200,81 @@ int timer_stop(gdb_ctx* ctx, double* ms) {
   return GDB_SUCCESS;
 }
 
+// ---- tuning table ------------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+std::vector<gdb_tune_entry> g_tune;
+bool g_tune_loaded = false;
+
+std::string default_tune_path() {
+  if (const char* env = getenv("GDB_TUNE_FILE")) return env;
+  Dl_info info;
+  if (dladdr((void*)&gdb_tune_load, &info) && info.dli_fname) {
+    std::string p = info.dli_fname;
+    const size_t slash = p.rfind('/');
+    return (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/tune_table.txt";
+  }
+  return "tune_table.txt";
+}
+
+// Parses the table; false on a malformed row.
+bool parse_tune(FILE* f, std::vector<gdb_tune_entry>* rows) {
+  char line[512];
+  while (fgets(line, sizeof(line), f)) {
+    char* hash = strchr(line, '#');
+    if (hash) *hash = 0;
+    gdb_tune_entry e{};
+    char name[64];
+    const int got = sscanf(line, "%63s %lld %d %d %d %d %d %d", name, &e.n_max, &e.wgs, &e.groups, &e.team_groups,
+                           &e.delta, &e.hops, &e.lds_queue);
+    if (got <= 0) continue;  // blank or comment line
+    if (got != 8 || strlen(name) >= sizeof(e.solver) || e.n_max < 0 || e.groups < 0 || e.team_groups < 0 ||
+        e.delta < 0 || e.hops < 0 || e.lds_queue < 0 ||
+        (e.wgs != 64 && e.wgs != 128 && e.wgs != 256 && e.wgs != 512 && e.wgs != 1024))
+      return false;
+    memcpy(e.solver, name, strlen(name) + 1);
+    rows->push_back(e);
+  }
+  return true;
+}
+
+int load_tune_locked(const char* path) {
+  const std::string p = path ? std::string(path) : default_tune_path();
+  g_tune.clear();
+  g_tune_loaded = true;
+  FILE* f = fopen(p.c_str(), "r");
+  if (!f) return path ? GDB_E_INVALID : 0;  // no default table: defaults stand
+  std::vector<gdb_tune_entry> rows;
+  const bool ok = parse_tune(f, &rows);
+  fclose(f);
+  if (!ok) return GDB_E_INVALID;
+  g_tune = rows;
+  return (int)g_tune.size();
+}
+
+bool lookup_tune_locked(const char* solver, long long n, gdb_tune_entry* out) {
+  const gdb_tune_entry* best = nullptr;
+  for (const gdb_tune_entry& e : g_tune)
+    if (!strcmp(e.solver, solver) && e.n_max >= n && (!best || e.n_max < best->n_max)) best = &e;
+  if (best) *out = *best;
+  return best != nullptr;
+}
+}  // namespace
+
+void apply_tuning(const char* solver, long long n, gdb_opts* o, int* team_groups) {
+  if (o->wgs != 0) return;
+  gdb_tune_entry e{};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (!g_tune_loaded) load_tune_locked(nullptr);
+    if (!lookup_tune_locked(solver, n, &e)) return;
+  }
+  o->wgs = e.wgs;
+  if (o->groups == 0) o->groups = e.groups;
+  if (o->lds_queue == 0) o->lds_queue = e.lds_queue;
+  if (team_groups && *team_groups == 0) *team_groups = e.team_groups;
+}
+
 static int init_ctx(gdb_ctx* ctx, int skip) {
   ctx->skip = skip;
   const int nlines = (int)(sizeof(gdb_kernel_ctx_t) / GDB_LINE_BYTES);
@@ -267,9 +345,25 @@ int gdb_device_count(int* out) {
   return GDB_SUCCESS;
 }
 
+int gdb_tune_load(const char* path) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  return load_tune_locked(path);
+}
+
+int gdb_tune_lookup(const char* solver, long long n, gdb_tune_entry* out) {
+  if (!solver || !out) return GDB_E_INVALID;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  if (!g_tune_loaded) load_tune_locked(nullptr);
+  return lookup_tune_locked(solver, n, out) ? GDB_SUCCESS : GDB_E_INVALID;
+}
+
 int gdb_ctx_create(gdb_ctx** out) {
   if (!out) return GDB_E_INVALID;
   *out = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (!g_tune_loaded) load_tune_locked(nullptr);
+  }
   gdb_ctx* ctx = new gdb_ctx();
   GDB_HIP(hipGetDevice(&ctx->device));
   hipDeviceProp_t prop;

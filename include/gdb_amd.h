@@ -88,6 +88,8 @@ typedef struct {
   int variant;        /* GDB_VARIANT_*                                         */
   int barrier;        /* GDB_BARRIER_* or -1 for the context default           */
   int relaunch_check; /* relaunch schedule: rounds between host stop checks (0/1 = every round) */
+  int lds_queue;      /* batched engine: ints per LDS queue buffer of a group's local hops,
+                         256..4096 (0 = the tuning table's value, else 2048) */
 } gdb_opts;
 
 typedef struct {
@@ -182,6 +184,56 @@ int gdb_barrier_check(gdb_ctx* ctx, int wgs, int groups, int iters, long long* v
  * mode 0: back-to-back async launches, one final sync.
  * mode 1: reference host loop, blocking 4-byte readback after every launch. */
 int gdb_relaunch_bench(gdb_ctx* ctx, int wgs, int groups, int iters, int mode, double* ms);
+
+/* Memory-latency probes (no reference counterpart; they measure what the
+ * barrier and worklist designs are built on).  gdb_probe_chain: one lane
+ * follows a random cycle of `steps` dependent accesses, one per 128 B line,
+ * over a buffer of `bytes` bytes (fine != 0: fine-grained memory); kind 0 a
+ * plain load, 1 a returning agent-scope atomic add, 2 an agent-scope relaxed
+ * atomic load.  *ns_per_step = device time / steps after a warm-up pass.
+ * gdb_probe_atomics: every lane of `groups` x `wgs` does `iters` dependent
+ * returning atomic-min operations (bits 32 or 64) on random lines of a
+ * 128 MB buffer; *ns_per_step = device time / iters. */
+int gdb_probe_chain(gdb_ctx* ctx, int kind, size_t bytes, int fine, int steps, double* ns_per_step);
+int gdb_probe_atomics(gdb_ctx* ctx, int bits, int groups, int wgs, int iters, double* ns_per_step);
+/* Two groups (on one XCD when same_xcd != 0, else on two) hand a counter back
+ * and forth `iters` times each; *ns_per_handoff = device time / (2 iters).
+ * mode 0 relaxed store / relaxed poll, 1 release store / poll + acquire,
+ * 2 acq_rel fetch-add / poll + acquire; fine != 0 puts the counter in
+ * fine-grained memory.  xcds (2 ints, may be NULL) receive the groups' XCDs.
+ * The floor under any inter-group barrier: a barrier is at least one
+ * arrival and one release handoff. */
+int gdb_probe_pingpong(gdb_ctx* ctx, int mode, int fine, int same_xcd, int iters, double* ns_per_handoff,
+                       int* xcds);
+
+/* ---- launch-shape tuning table -----------------------------------------------
+ * Counterpart of the reference's tuned work-group-size table
+ * (pannotia/scripts/wg_size_data.py:7-79), written by its tuner
+ * (tune_wg_size.py:106-124) and read by its suite runner.  Here the tuner
+ * (python -m gpu_discovery_barrier_amd.autotune) times launch shapes on the
+ * device and writes a text table, one row per line:
+ *     solver n_max wgs groups team_groups delta hops lds_queue
+ * (solver: color, mis, bc, sssp_pull, bfs_wl, sssp_wl, mst, batch; '#'
+ * starts a comment).  The library loads it once per process, at the first
+ * gdb_ctx_create, from $GDB_TUNE_FILE or else tune_table.txt beside
+ * libgdb_amd.so.  A solver called with opts == NULL or opts->wgs == 0 takes
+ * wgs, groups (when opts->groups == 0) and lds_queue from the row for its
+ * solver with the smallest n_max >= n (the tuned persistent variant only;
+ * the plain variant keeps the reference's default launch); gdb_batch_solve
+ * also takes team_groups when it is 0, keyed by the largest task's n.
+ * Without a row the defaults stand (256 threads, the residency bound). */
+typedef struct {
+  char solver[16];
+  long long n_max;
+  int wgs, groups, team_groups, delta, hops, lds_queue;
+} gdb_tune_entry;
+/* Replaces the table with the file's rows (path NULL: the default path);
+ * returns the number of rows, or GDB_E_INVALID for an unreadable or
+ * malformed file (the table is then empty). */
+int gdb_tune_load(const char* path);
+/* The row a solver of graph size n would use: GDB_SUCCESS, or GDB_E_INVALID
+ * when there is none. */
+int gdb_tune_lookup(const char* solver, long long n, gdb_tune_entry* out);
 
 /* ---- solvers (device pointers; CSR int32, row has n+1 entries) ------------- */
 /* Pannotia colouring (max rule): color[v] = round in which v was coloured. */

@@ -48,6 +48,7 @@ def lib():
             "orc_bc_brandes_tb": (I, [I, P, P, P, I, P]),
             "orc_bc_pairs_tb": (I, [I, P, P, P, I, P]),
             "orc_bc_brandes_mt": (I, [I, P, P, P, I, I, P]),
+            "orc_worklist_mt": (I, [I, P, P, P, I, I, P, C.POINTER(I)]),
             "orc_bc_ref_f32": (I, [I, P, P, P, P, P, I, P]),
             "orc_block_scan_ref": (I, [I, P, P]),
             "orc_dp_model": (I, [I, I, I, I, I, I, I, C.POINTER(I), P, C.POINTER(C.c_longlong)]),
@@ -143,6 +144,23 @@ def sssp_worklist_ref(row, col, w, src=0):
     out = np.empty(n, np.uint32)
     it = lib().orc_sssp_worklist_ref(n, _p(row), _p(col), _p(w), src, _p(out))
     return out, it
+
+
+def worklist_mt(row, col, w, src=0, threads=None):
+    """Worklist BFS (w None) or SSSP on `threads` host threads with a CPU XF
+    barrier: (dist uint32[n], iterations)."""
+    import os
+    row, col = _i32(row), _i32(col)
+    w = None if w is None else _i32(w)
+    threads = threads or min(16, os.cpu_count() or 1)
+    n = row.size - 1
+    out = np.empty(n, np.uint32)
+    it = C.c_int(0)
+    rc = lib().orc_worklist_mt(n, _p(row), _p(col), None if w is None else _p(w), src, int(threads), _p(out),
+                               C.byref(it))
+    if rc:
+        raise MemoryError("orc_worklist_mt")
+    return out, it.value
 
 
 def bfs_tb(row, col, src=0):

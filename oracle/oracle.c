@@ -365,6 +365,182 @@ int orc_sssp_worklist_ref(int n, const int* row, const int* col, const int* w, i
   return iters;
 }
 
+/* ---- multi-threaded worklist BFS / SSSP (the CPU baseline) ---------------
+ * The same worklist iterations as orc_bfs_worklist_ref / orc_sssp_worklist_ref
+ * (LS/apps/bfs/kernel/bfs_port_kernel.cl:146-192, sssp_port_kernel.cl:152-197)
+ * run by `threads` persistent pthreads, one per core: each iteration the
+ * threads take the input worklist with a grid stride (item t, t + T, ...),
+ * relax edges with atomic compare-and-swap (BFS claims an unvisited vertex,
+ * SSSP lowers the distance), buffer their pushes and reserve output space
+ * with one fetch-add per 1024 items, then meet at a CPU restatement of the
+ * XF barrier (DP/api/discovery.cl:129-190: the master waits for every
+ * slave's flag, swaps the worklists inside the barrier, then clears the
+ * flags).  SSSP dedupes pushes per iteration with a stamp. */
+typedef struct {
+  _Atomic int v;
+  char pad[60];
+} orc_pad_int;
+
+typedef struct {
+  int n, T;
+  const int *row, *col, *w;
+  _Atomic uint32_t* dist;
+  _Atomic int* stamp;
+  int* q[2];
+  orc_pad_int out_size;
+  orc_pad_int* flags;
+  int nin, iter, cur; /* written by the master inside the barrier */
+  int iterations;
+} orc_wl_shared;
+
+typedef struct {
+  orc_wl_shared* S;
+  int t;
+} orc_wl_arg;
+
+static void orc_spin_pause(unsigned* k) {
+  if ((++*k & 1023u) == 0) sched_yield();
+#if defined(__x86_64__) || defined(__i386__)
+  else __builtin_ia32_pause();
+#endif
+}
+
+/* XF barrier; `serial` runs on the master between the arrivals and the release. */
+static void orc_xf_barrier(orc_wl_shared* S, int t, void (*serial)(orc_wl_shared*)) {
+  if (t == 0) {
+    for (int p = 1; p < S->T; ++p) {
+      unsigned k = 0;
+      while (atomic_load_explicit(&S->flags[p].v, memory_order_acquire) == 0) orc_spin_pause(&k);
+    }
+    serial(S);
+    for (int p = 1; p < S->T; ++p) atomic_store_explicit(&S->flags[p].v, 0, memory_order_release);
+  } else {
+    atomic_store_explicit(&S->flags[t].v, 1, memory_order_release);
+    unsigned k = 0;
+    while (atomic_load_explicit(&S->flags[t].v, memory_order_acquire) == 1) orc_spin_pause(&k);
+  }
+}
+
+static void orc_wl_swap(orc_wl_shared* S) {
+  S->nin = atomic_load_explicit(&S->out_size.v, memory_order_relaxed);
+  atomic_store_explicit(&S->out_size.v, 0, memory_order_relaxed);
+  S->cur ^= 1;
+  S->iter += 1;
+  if (S->nin > 0) S->iterations += 1;
+}
+
+static void orc_wl_flush(orc_wl_shared* S, int* out, const int* buf, int cnt) {
+  if (cnt == 0) return;
+  const int at = atomic_fetch_add_explicit(&S->out_size.v, cnt, memory_order_relaxed);
+  memcpy(out + at, buf, sizeof(int) * (size_t)cnt);
+}
+
+static void* orc_wl_worker(void* p) {
+  orc_wl_arg* a = (orc_wl_arg*)p;
+  orc_wl_shared* S = a->S;
+  const int t = a->t, T = S->T;
+  int buf[1024];
+  for (;;) {
+    const int nin = S->nin, it = S->iter;
+    if (nin == 0) break;
+    const int* in = S->q[S->cur];
+    int* out = S->q[S->cur ^ 1];
+    int cnt = 0;
+    for (int i = t; i < nin; i += T) {
+      const int v = in[i];
+      const uint32_t dv = atomic_load_explicit(&S->dist[v], memory_order_relaxed);
+      for (int e = S->row[v]; e < S->row[v + 1]; ++e) {
+        const int dst = S->col[e];
+        if (dst < 0 || dst >= S->n) continue;
+        int push = 0;
+        if (!S->w) { /* processedge2, bfs_common.cl:73-90 */
+          uint32_t expect = ORC_LONESTAR_INF;
+          if (atomic_load_explicit(&S->dist[dst], memory_order_relaxed) == ORC_LONESTAR_INF)
+            push = atomic_compare_exchange_strong_explicit(&S->dist[dst], &expect, (uint32_t)it,
+                                                           memory_order_relaxed, memory_order_relaxed);
+        } else { /* processedge2, sssp_common.cl:58-81 */
+          const uint32_t wt = (uint32_t)S->w[e];
+          if (wt >= ORC_LONESTAR_INF) continue;
+          const uint32_t alt = dv + wt;
+          uint32_t cur = atomic_load_explicit(&S->dist[dst], memory_order_relaxed);
+          while (alt < cur) {
+            if (atomic_compare_exchange_weak_explicit(&S->dist[dst], &cur, alt, memory_order_relaxed,
+                                                      memory_order_relaxed)) {
+              push = atomic_exchange_explicit(&S->stamp[dst], it, memory_order_relaxed) != it;
+              break;
+            }
+          }
+        }
+        if (push) {
+          buf[cnt++] = dst;
+          if (cnt == 1024) {
+            orc_wl_flush(S, out, buf, cnt);
+            cnt = 0;
+          }
+        }
+      }
+    }
+    orc_wl_flush(S, out, buf, cnt);
+    orc_xf_barrier(S, t, orc_wl_swap);
+  }
+  return NULL;
+}
+
+int orc_worklist_mt(int n, const int* row, const int* col, const int* w, int src, int threads, uint32_t* dist,
+                    int* iterations) {
+  if (threads < 1) threads = 1;
+  if (n <= 0) {
+    if (iterations) *iterations = 0;
+    return 0;
+  }
+  orc_wl_shared S;
+  memset(&S, 0, sizeof(S));
+  S.n = n;
+  S.T = threads;
+  S.row = row;
+  S.col = col;
+  S.w = w;
+  S.dist = (_Atomic uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+  S.stamp = (_Atomic int*)malloc(sizeof(int) * (size_t)n);
+  S.q[0] = (int*)malloc(sizeof(int) * (size_t)n);
+  S.q[1] = (int*)malloc(sizeof(int) * (size_t)n);
+  S.flags = (orc_pad_int*)calloc((size_t)threads, sizeof(orc_pad_int));
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+  orc_wl_arg* args = (orc_wl_arg*)malloc(sizeof(orc_wl_arg) * (size_t)threads);
+  if (!S.dist || !S.stamp || !S.q[0] || !S.q[1] || !S.flags || !th || !args) return -2;
+  for (int i = 0; i < n; ++i) {
+    atomic_init(&S.dist[i], ORC_LONESTAR_INF);
+    atomic_init(&S.stamp[i], 0);
+  }
+  atomic_init(&S.dist[src], 0u);
+  atomic_init(&S.out_size.v, 0);
+  for (int p = 0; p < threads; ++p) atomic_init(&S.flags[p].v, 0);
+  S.q[0][0] = src; /* iteration 0 pushes the source (bfs_port_kernel.cl:130-137) */
+  S.nin = 1;
+  S.iter = 1;
+  S.cur = 0;
+  S.iterations = 1;
+  for (int t = 1; t < threads; ++t) {
+    args[t].S = &S;
+    args[t].t = t;
+    pthread_create(&th[t], NULL, orc_wl_worker, &args[t]);
+  }
+  args[0].S = &S;
+  args[0].t = 0;
+  orc_wl_worker(&args[0]);
+  for (int t = 1; t < threads; ++t) pthread_join(th[t], NULL);
+  for (int i = 0; i < n; ++i) dist[i] = atomic_load_explicit(&S.dist[i], memory_order_relaxed);
+  if (iterations) *iterations = S.iterations;
+  free((void*)S.dist);
+  free((void*)S.stamp);
+  free(S.q[0]);
+  free(S.q[1]);
+  free(S.flags);
+  free(th);
+  free(args);
+  return 0;
+}
+
 int orc_bfs_tb(int n, const int* row, const int* col, int src, uint32_t* dist) {
   for (int i = 0; i < n; ++i) dist[i] = ORC_LONESTAR_INF;
   if (n == 0) return 0;
@@ -656,6 +832,7 @@ typedef struct model {
   line_t poll, prot_poll_open, prot_counter, kernel_counter, num_participating, closing_p;
   line_t lock_counter, now_serving, bar_count, bar_gen, cluster[64], rounds[3];
   line_t* flags;
+  int* data; /* plain payload: 2 x groups ints, written before each barrier, read after */
   _Atomic long long progress;
   _Atomic int abort_flag;
   _Atomic long long violations;
@@ -675,6 +852,23 @@ static double now_ms(void) {
   return ts.tv_sec * 1e3 + ts.tv_nsec / 1e6;
 }
 
+/* Memory orders follow the device code (include/gdb_discovery.hpp): polls
+ * are relaxed, protocol and barrier read-modify-writes are acq_rel, the
+ * publishing stores are release.  The device's "relaxed poll loop, then an
+ * acquire fence" and "release fence, then a relaxed store" appear here as an
+ * acquire poll and a release store on the same word: the same
+ * synchronises-with edges, in a form ThreadSanitizer checks (it does not
+ * model standalone fences).  g_weak plants a mistake for the tests: the
+ * barrier's arrival and release operations become relaxed, and the plain
+ * payload written before each barrier must then race with its readers. */
+static int g_weak = 0;
+void orc_dp_model_set_weak(int weak) { g_weak = weak; }
+#define ORD_REL (g_weak ? memory_order_relaxed : memory_order_release)
+#define ORD_ACQ (g_weak ? memory_order_relaxed : memory_order_acquire)
+#define ORD_AR (g_weak ? memory_order_relaxed : memory_order_acq_rel)
+#define LD(p, o) atomic_load_explicit((p), (o))
+#define ST(p, v, o) atomic_store_explicit((p), (v), (o))
+
 /* Spin helper: returns 0 if aborted. */
 #define SPIN_WHILE(m, cond)                              \
   do {                                                   \
@@ -685,27 +879,26 @@ static double now_ms(void) {
     atomic_fetch_add(&(m)->progress, 1);                 \
   } while (0)
 
-/* DP/locks/locks.cl:11-34. */
+/* DP/locks/locks.cl:11-34 with the device's orders (lock_acquire /
+ * lock_release in gdb_discovery.hpp). */
 static int m_lock(model_t* m) {
   if (m->lock == 2) {
-    int exp;
     for (;;) {
-      exp = 0;
-      if (atomic_compare_exchange_weak(&m->lock_counter.v, &exp, 1)) break;
+      if (atomic_exchange_explicit(&m->lock_counter.v, 1, memory_order_acq_rel) == 0) break;
       if (atomic_load(&m->abort_flag)) return 0;
       sched_yield();
     }
     return 1;
   }
-  const int ticket = atomic_fetch_add(&m->lock_counter.v, 1);
-  SPIN_WHILE(m, atomic_load(&m->now_serving.v) != ticket);
+  const int ticket = atomic_fetch_add_explicit(&m->lock_counter.v, 1, memory_order_acq_rel);
+  SPIN_WHILE(m, LD(&m->now_serving.v, memory_order_acquire) != ticket);
   return 1;
 }
 static void m_unlock(model_t* m) {
   if (m->lock == 2)
-    atomic_store(&m->lock_counter.v, 0);
+    ST(&m->lock_counter.v, 0, memory_order_release);
   else
-    atomic_store(&m->now_serving.v, atomic_load(&m->now_serving.v) + 1);
+    ST(&m->now_serving.v, LD(&m->now_serving.v, memory_order_relaxed) + 1, memory_order_release);
 }
 
 /* discovery_protocol_master, DP/api/discovery.cl:42-88, with a mutex.
@@ -713,40 +906,40 @@ static void m_unlock(model_t* m) {
 static int m_protocol_locked(model_t* m, int* id, int* P) {
   *id = -1;
   if (!m_lock(m)) return 0;
-  if (atomic_load(&m->prot_poll_open.v)) {
-    *id = atomic_load(&m->prot_counter.v);
-    atomic_store(&m->prot_counter.v, *id + 1);
+  if (LD(&m->prot_poll_open.v, memory_order_relaxed)) {
+    *id = LD(&m->prot_counter.v, memory_order_relaxed);
+    ST(&m->prot_counter.v, *id + 1, memory_order_relaxed);
   }
   m_unlock(m);
   if (!m_lock(m)) return 0;
-  const int kc = atomic_load(&m->kernel_counter.v) + 1;
-  atomic_store(&m->kernel_counter.v, kc);
-  if (atomic_load(&m->prot_poll_open.v)) {
-    atomic_store(&m->prot_poll_open.v, 0);
-    atomic_store(&m->num_participating.v, atomic_load(&m->prot_counter.v));
+  const int kc = LD(&m->kernel_counter.v, memory_order_relaxed) + 1;
+  ST(&m->kernel_counter.v, kc, memory_order_relaxed);
+  if (LD(&m->prot_poll_open.v, memory_order_relaxed)) {
+    ST(&m->prot_poll_open.v, 0, memory_order_relaxed);
+    ST(&m->num_participating.v, LD(&m->prot_counter.v, memory_order_relaxed), memory_order_relaxed);
   }
-  *P = atomic_load(&m->prot_counter.v);
-  if (kc == m->groups) { /* reset_kernel_context, discovery.cl:31-37 */
-    atomic_store(&m->prot_poll_open.v, 1);
-    atomic_store(&m->prot_counter.v, 0);
-    atomic_store(&m->kernel_counter.v, 0);
-    atomic_store(&m->now_serving.v, 0);
-    atomic_store(&m->lock_counter.v, 0);
+  *P = LD(&m->prot_counter.v, memory_order_relaxed);
+  if (kc == m->groups) { /* reset_kernel_context, discovery.cl:31-37; the last store unlocks */
+    ST(&m->prot_poll_open.v, 1, memory_order_relaxed);
+    ST(&m->prot_counter.v, 0, memory_order_relaxed);
+    ST(&m->kernel_counter.v, 0, memory_order_relaxed);
+    ST(&m->now_serving.v, 0, memory_order_relaxed);
+    ST(&m->lock_counter.v, 0, memory_order_release);
   } else {
     m_unlock(m);
   }
   return 1;
 }
 
-/* Lock-free variant (same as the device's protocol_lockfree). */
+/* Lock-free variant (protocol_lockfree in gdb_discovery.hpp). */
 static int m_protocol_lockfree(model_t* m, int* id, int* P) {
-  const int old = atomic_fetch_add(&m->poll.v, 1);
+  const int old = atomic_fetch_add_explicit(&m->poll.v, 1, memory_order_acq_rel);
   const int part = (old & CLOSED) == 0;
   if (part) {
     int last = old + 1;
     double t_last = now_ms();
     for (;;) {
-      const int cur = atomic_load(&m->poll.v);
+      const int cur = LD(&m->poll.v, memory_order_relaxed);
       if (cur & CLOSED) break;
       if ((cur & MASK) >= m->groups) break;
       const double t = now_ms();
@@ -760,72 +953,74 @@ static int m_protocol_lockfree(model_t* m, int* id, int* P) {
       sched_yield();
     }
   }
-  const int o2 = atomic_fetch_or(&m->poll.v, CLOSED);
+  const int o2 = atomic_fetch_or_explicit(&m->poll.v, CLOSED, memory_order_acq_rel);
   int p = 0;
   if ((o2 & CLOSED) == 0) {
     p = o2 & MASK;
-    atomic_store(&m->num_participating.v, p);
-    atomic_store(&m->closing_p.v, p);
+    ST(&m->num_participating.v, p, memory_order_relaxed);
+    ST(&m->closing_p.v, p, memory_order_release);
   } else if (part) {
-    SPIN_WHILE(m, atomic_load(&m->closing_p.v) == 0);
-    p = atomic_load(&m->closing_p.v);
+    SPIN_WHILE(m, LD(&m->closing_p.v, memory_order_acquire) == 0);
+    p = LD(&m->closing_p.v, memory_order_acquire);
   }
   *id = part ? (old & MASK) : -1;
   *P = p;
-  const int kc = atomic_fetch_add(&m->kernel_counter.v, 1);
+  const int kc = atomic_fetch_add_explicit(&m->kernel_counter.v, 1, memory_order_acq_rel);
   if (kc == m->groups - 1) {
-    atomic_store(&m->poll.v, 0);
-    atomic_store(&m->closing_p.v, 0);
-    atomic_store(&m->kernel_counter.v, 0);
+    ST(&m->poll.v, 0, memory_order_relaxed);
+    ST(&m->closing_p.v, 0, memory_order_relaxed);
+    ST(&m->kernel_counter.v, 0, memory_order_release);
   }
   return 1;
 }
 
 /* XF_barrier, DP/api/discovery.cl:129-190, one representative per group
- * (the master's per-thread loop over slaves is run by one thread here). */
+ * (the master's per-thread loop over slaves is run by one thread here);
+ * orders as xf_barrier in gdb_discovery.hpp. */
 static int m_xf_barrier(model_t* m, int id, int P) {
   if (id == 0) {
-    for (int peer = 1; peer < P; ++peer) SPIN_WHILE(m, atomic_load(&m->flags[peer].v) == 0);
-    for (int peer = 1; peer < P; ++peer) atomic_store(&m->flags[peer].v, 0);
+    for (int peer = 1; peer < P; ++peer) SPIN_WHILE(m, LD(&m->flags[peer].v, ORD_ACQ) == 0);
+    for (int peer = 1; peer < P; ++peer) ST(&m->flags[peer].v, 0, ORD_REL);
   } else {
-    atomic_store(&m->flags[id].v, 1);
-    SPIN_WHILE(m, atomic_load(&m->flags[id].v) == 1);
+    ST(&m->flags[id].v, 1, ORD_REL);
+    SPIN_WHILE(m, LD(&m->flags[id].v, ORD_ACQ) == 1);
   }
   return 1;
 }
 
+/* counter_wait in gdb_discovery.hpp. */
 static int m_counter_wait(model_t* m, _Atomic int* count, int members, int* gen_cache) {
   const int g = *gen_cache;
-  const int old = atomic_fetch_add(count, 1);
+  const int old = atomic_fetch_add_explicit(count, 1, ORD_AR);
   if (old == members - 1) {
-    atomic_store(count, 0);
-    atomic_store(&m->bar_gen.v, g + 1);
+    ST(count, 0, memory_order_relaxed);
+    ST(&m->bar_gen.v, g + 1, ORD_REL);
   } else {
-    SPIN_WHILE(m, atomic_load(&m->bar_gen.v) == g);
+    SPIN_WHILE(m, LD(&m->bar_gen.v, ORD_ACQ) == g);
   }
   *gen_cache = g + 1;
   return 1;
 }
 
+/* tree_barrier in gdb_discovery.hpp: clusters of consecutive ids. */
 static int m_tree_barrier(model_t* m, int id, int P, int* gen_cache) {
-  int C = (P + 63) / 64;
-  if (C < 16) C = 16;
+  const int C = (P + 15) / 16;
   const int c = id / C;
   const int members = (P - c * C) < C ? (P - c * C) : C;
   const int nclus = (P + C - 1) / C;
   const int g = *gen_cache;
-  const int old = atomic_fetch_add(&m->cluster[c].v, 1);
+  const int old = atomic_fetch_add_explicit(&m->cluster[c].v, 1, ORD_AR);
   if (old == members - 1) {
-    atomic_store(&m->cluster[c].v, 0);
-    const int o2 = atomic_fetch_add(&m->bar_count.v, 1);
+    ST(&m->cluster[c].v, 0, memory_order_relaxed);
+    const int o2 = atomic_fetch_add_explicit(&m->bar_count.v, 1, ORD_AR);
     if (o2 == nclus - 1) {
-      atomic_store(&m->bar_count.v, 0);
-      atomic_store(&m->bar_gen.v, g + 1);
+      ST(&m->bar_count.v, 0, memory_order_relaxed);
+      ST(&m->bar_gen.v, g + 1, ORD_REL);
       *gen_cache = g + 1;
       return 1;
     }
   }
-  SPIN_WHILE(m, atomic_load(&m->bar_gen.v) == g);
+  SPIN_WHILE(m, LD(&m->bar_gen.v, ORD_ACQ) == g);
   *gen_cache = g + 1;
   return 1;
 }
@@ -858,9 +1053,14 @@ static void* m_group(void* arg) {
   }
   m->ids[a->gid] = ok ? id : -1;
   if (ok && id >= 0) {
-    int gen = atomic_load(&m->bar_gen.v);
+    int gen = atomic_load_explicit(&m->bar_gen.v, memory_order_acquire);
+    const int nb = (id + 1) % P;
     for (int k = 0; k < m->barriers && ok; ++k) {
-      atomic_fetch_add(&m->rounds[k % 3].v, 1);
+      /* Before barrier k: a plain payload store and a relaxed count (as
+       * gdb_k_barrier_check); after it both must be visible. */
+      int* slot = m->data + (size_t)(k & 1) * (size_t)m->groups;
+      slot[id] = k * 7919 + id;
+      atomic_fetch_add_explicit(&m->rounds[k % 3].v, 1, memory_order_relaxed);
       if (P > 1) {
         if (m->bar == 0)
           ok = m_xf_barrier(m, id, P);
@@ -869,7 +1069,8 @@ static void* m_group(void* arg) {
         else
           ok = m_tree_barrier(m, id, P, &gen);
       }
-      if (ok && atomic_load(&m->rounds[k % 3].v) != P * (k / 3 + 1))
+      if (ok && (atomic_load_explicit(&m->rounds[k % 3].v, memory_order_relaxed) != P * (k / 3 + 1) ||
+                 slot[nb] != k * 7919 + nb))
         atomic_fetch_add(&m->violations, 1);
     }
   }
@@ -891,6 +1092,7 @@ int orc_dp_model(int groups, int resident, int lock, int bar, int skip, int barr
   m->skip = skip;
   m->barriers = barriers;
   m->flags = (line_t*)calloc((size_t)groups, sizeof(line_t));
+  m->data = (int*)calloc((size_t)groups * 2, sizeof(int));
   m->ids = ids;
   atomic_store(&m->prot_poll_open.v, 1);
   sem_init(&m->slots, 0, (unsigned)resident);
@@ -903,7 +1105,7 @@ int orc_dp_model(int groups, int resident, int lock, int bar, int skip, int barr
     ids[g] = -1;
     pthread_attr_t attr;
     pthread_attr_init(&attr);
-    pthread_attr_setstacksize(&attr, 64 * 1024);
+    pthread_attr_setstacksize(&attr, 256 * 1024);
     pthread_create(&th[g], &attr, m_group, &args[g]);
     pthread_attr_destroy(&attr);
   }
@@ -929,6 +1131,7 @@ int orc_dp_model(int groups, int resident, int lock, int bar, int skip, int barr
   free(th);
   free(args);
   free(m->flags);
+  free(m->data);
   free(m);
   return deadlock;
 }

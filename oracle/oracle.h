@@ -52,6 +52,11 @@ int orc_bellman_ford_tb(int n, const int* row, const int* col, const int* w, int
 int orc_bfs_worklist_ref(int n, const int* row, const int* col, int src, uint32_t* dist);
 int orc_sssp_worklist_ref(int n, const int* row, const int* col, const int* w, int src,
                           uint32_t* dist);
+/* The same worklist iterations on `threads` persistent pthreads with a CPU
+ * XF barrier (the CPU baseline): w == NULL gives BFS levels, else SSSP
+ * distances; *iterations = worklist iterations with a non-empty input. */
+int orc_worklist_mt(int n, const int* row, const int* col, const int* w, int src, int threads, uint32_t* dist,
+                    int* iterations);
 /* Queue BFS. */
 int orc_bfs_tb(int n, const int* row, const int* col, int src, uint32_t* dist);
 /* dverifysolution: edges with dist[u] + w < dist[v] (w = 1 if w == NULL). */
@@ -90,6 +95,13 @@ int orc_block_scan_ref(int n_items, const int* in, int* out);
  * progress for `watchdog_ms` (deadlock). */
 int orc_dp_model(int groups, int resident, int lock, int bar, int skip, int barriers,
                  int watchdog_ms, int* participating, int* ids, long long* violations);
+/* The model uses the device's memory orders (relaxed polls, acq_rel
+ * read-modify-writes, release publication), with each device fence carried
+ * by the atomic it pairs with, and a plain payload written before every
+ * barrier and read after it, so a ThreadSanitizer build reports any missing
+ * synchronisation.  weak != 0 plants a mistake for the tests: the barrier's
+ * arrival and release operations become relaxed. */
+void orc_dp_model_set_weak(int weak);
 
 /* ---- schedule selector MLP (FP64) ----------------------------------------
  * No reference counterpart.  16-64-64-4 ReLU network, parameters in the

@@ -86,3 +86,42 @@ def test_task_keeps_its_device_objects_alive():
     del t
     gc.collect()
     assert all(r() is None for r in refs)
+
+
+def test_tuning_table_load_lookup_and_rejects_malformed(tmp_path):
+    # The table the autotuner writes and every solver consults when called
+    # with wgs = 0: the row for the solver with the smallest n_max >= n.
+    from gpu_discovery_barrier_amd import autotune
+
+    t = tmp_path / "tune.txt"
+    autotune.write_table([autotune.row("batch", 1000, 128, 8, 4, 50, 16, 512),
+                          autotune.row("batch", 1 << 30, 256, 64, 32, 4096, 256, 2048),
+                          autotune.row("color", 5000, 64, 16)], t, header="test")
+    try:
+        assert gdb.tune_load(str(t)) == 3
+        assert gdb.tune_lookup("batch", 999)["groups"] == 8
+        assert gdb.tune_lookup("batch", 1000)["lds_queue"] == 512
+        assert gdb.tune_lookup("batch", 1001)["wgs"] == 256
+        assert gdb.tune_lookup("color", 6000) is None
+        assert gdb.tune_lookup("mis", 10) is None
+        assert autotune.lookup("batch", 1001, t)["team_groups"] == 32  # the Python reader agrees
+        bad = tmp_path / "bad.txt"
+        bad.write_text("batch 100 100 8 1 0 0 0\n")  # wgs 100 is not a supported size
+        import pytest
+
+        with pytest.raises(gdb.GdbError):
+            gdb.tune_load(str(bad))
+        assert gdb.tune_lookup("batch", 10) is None  # a rejected file leaves the table empty
+    finally:
+        gdb.tune_load(None)  # back to the default table beside the library
+    assert gdb.tune_lookup("batch", 2048 * 2048) == autotune.lookup("batch", 2048 * 2048)
+
+
+def test_committed_tuning_table_is_well_formed():
+    from gpu_discovery_barrier_amd import autotune
+
+    rows = autotune.read_table()
+    assert {r["solver"] for r in rows} >= {"batch", "color", "mis", "bc"}
+    for r in rows:
+        assert r["wgs"] in (64, 128, 256, 512, 1024)
+        assert r["lds_queue"] == 0 or 256 <= r["lds_queue"] <= 4096

@@ -211,3 +211,49 @@ def test_batch_color_blockers(ctx, team, order):
         want, rr = oracle.color_ref(g.row, g.col, v)
         assert (out.to_numpy() == want).all()
         assert r == rr
+
+
+def test_batch_shape_and_lds_tile_from_tuning_table(ctx, tmp_path):
+    # opts.wgs = 0: the launch shape, team size and LDS queue tile come from
+    # the tuning table; a 256-int tile makes local hops spill early.
+    from gpu_discovery_barrier_amd import autotune
+
+    t = tmp_path / "tune.txt"
+    autotune.write_table([autotune.row("batch", 1 << 20, 128, 8, 4, 0, 0, 256)], t)
+    gs = [gdb.gen_grid(200, 200, 0.05, seed=2, wlo=1, whi=255), gdb.gen_rmat(12, 16, seed=7, wlo=1, whi=255)]
+    try:
+        gdb.tune_load(str(t))
+        kinds, outs, vals, rounds, st = run_batch(ctx, gs, 0, 500, wgs=0, groups=0, hops=64)
+    finally:
+        gdb.tune_load(None)
+    print(st.as_dict())
+    assert 1 <= st.participating <= 8
+    check(gs, kinds, outs, vals, rounds, label_correcting=True)
+
+
+@pytest.mark.parametrize("qcap", [256, 1024, 4096])
+def test_batch_lds_queue_tiles_are_exact(ctx, qcap):
+    gs = [gdb.gen_grid(160, 160, 0.05, seed=3, wlo=1, whi=255), gdb.gen_rmat(14, 16, seed=11, wlo=1, whi=255)]
+    dgs, outs, tasks = [], [], []
+    for g in gs:
+        dg = gdb.DeviceCSR(g)
+        dgs.append(dg)
+        ob, os_ = gdb.DeviceBuffer.empty(g.n, np.uint32), gdb.DeviceBuffer.empty(g.n, np.uint32)
+        outs.append((ob, os_))
+        tasks += [gdb.make_task(gdb.TASK_BFS, dg, ob, 0, hops=256), gdb.make_task(gdb.TASK_SSSP, dg, os_, 0, delta=1024,
+                                                                                  hops=256)]
+    gdb.batch_solve(ctx, tasks, 8, gdb.Opts.make(wgs=128, groups=32, lds_queue=qcap))
+    for g, (ob, os_) in zip(gs, outs):
+        assert (ob.to_numpy() == oracle.bfs_tb(g.row, g.col, 0)).all()
+        assert (os_.to_numpy().astype(np.int64) == oracle.dijkstra_tb(g.row, g.col, g.w, 0)).all()
+
+
+@pytest.mark.parametrize("qcap", [100, 8192])
+def test_batch_rejects_lds_queue_outside_range(ctx, qcap):
+    g = gdb.gen_grid(8, 8, 0.0, seed=1, wlo=1, whi=9)
+    dg = gdb.DeviceCSR(g)
+    out = gdb.DeviceBuffer.empty(g.n, np.uint32)
+    with pytest.raises(gdb.GdbError) as e:
+        gdb.batch_solve(ctx, [gdb.make_task(gdb.TASK_BFS, dg, out, 0)], 1, gdb.Opts.make(wgs=128, groups=2,
+                                                                                      lds_queue=qcap))
+    assert e.value.code == gdb.GDB_E_INVALID

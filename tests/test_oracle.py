@@ -167,6 +167,30 @@ def test_bc_brandes_matches_pair_definition():
     np.testing.assert_allclose(f, a, rtol=1e-5, atol=1e-4)
 
 
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_worklist_threaded_matches_queue_bfs_and_dijkstra(threads):
+    # The threaded CPU baseline (persistent threads, CPU XF barrier) against
+    # queue BFS and Dijkstra, and its BFS iteration count against the
+    # restatement's (levels + 1 on a connected graph).
+    for g in (gdb.gen_grid(70, 90, 0.05, seed=2, wlo=1, whi=255), gdb.gen_rmat(12, 16, seed=6, wlo=1, whi=255)):
+        for src in (0, 17):
+            got, it = oracle.worklist_mt(g.row, g.col, None, src, threads=threads)
+            assert (got == oracle.bfs_tb(g.row, g.col, src)).all()
+            want, it_ref = oracle.bfs_worklist_ref(g.row, g.col, src)
+            assert (got == want).all() and it == it_ref
+            got_s, _ = oracle.worklist_mt(g.row, g.col, g.w, src, threads=threads)
+            assert (got_s.astype(np.int64) == oracle.dijkstra_tb(g.row, g.col, g.w, src)).all()
+
+
+def test_worklist_threaded_planted_mistake_is_caught():
+    # A distance one too large at a reachable vertex fails the Dijkstra pin.
+    g = gdb.gen_grid(30, 30, 0.0, seed=1, wlo=1, whi=9)
+    got, _ = oracle.worklist_mt(g.row, g.col, g.w, 0, threads=4)
+    bad = got.copy()
+    bad[400] += 1
+    assert not (bad.astype(np.int64) == oracle.dijkstra_tb(g.row, g.col, g.w, 0)).all()
+
+
 def test_bc_brandes_threaded_matches_pair_definition():
     # The threaded form (sources dealt to threads, accumulators summed in
     # thread order) against the pair-dependency definition and the
