@@ -441,6 +441,43 @@ def bench_probe(args) -> int:
     return 0
 
 
+def bench_sched(args) -> int:
+    """Persistent barrier vs kernel relaunch as the schedule of a worklist
+    solve: BFS and SSSP on the 2048^2 grid (thousands of short iterations)
+    and on R-MAT 20 (a dozen wide ones), each run (a) persistent tuned, one
+    inter-group barrier per iteration, and (b) relaunched per iteration with
+    the host reading the stop flag after every 1, 4 or 16 launches (the
+    reference's color.cpp:294-334 loop and its batched variant).  Reported:
+    kernel ms and us per iteration; outputs compared bit for bit."""
+    import gpu_discovery_barrier_amd as gdb
+
+    ctx = gdb.Context()
+    rows = []
+    for name, g in (("grid2048", gdb.gen_grid(GRID_SIDE, GRID_SIDE, 0.05, seed=2, wlo=1, whi=255)),
+                    ("rmat20", gdb.gen_rmat(20, 16, seed=50, wlo=1, whi=255))):
+        dg = gdb.DeviceCSR(g)
+        for algo, fn in (("bfs", gdb.bfs_worklist), ("sssp", gdb.sssp_worklist)):
+            ref, st = fn(ctx, dg, 0, gdb.Opts.make(variant=gdb.VARIANT_TUNED, wgs=512, groups=128))
+            ms = min(fn(ctx, dg, 0, gdb.Opts.make(variant=gdb.VARIANT_TUNED, wgs=512, groups=128))[1].kernel_ms
+                     for _ in range(2))
+            r = {"graph": name, "algo": algo, "iterations": st.rounds,
+                 "persistent": {"ms": round(ms, 3), "us_per_iteration": round(ms * 1e3 / max(1, st.rounds), 3)}}
+            for check in (1, 4, 16):
+                o = gdb.Opts.make(schedule=gdb.SCHED_RELAUNCH, wgs=256, relaunch_check=check)
+                got, s2 = fn(ctx, dg, 0, o)
+                r[f"relaunch_check{check}"] = {"ms": round(s2.kernel_ms, 3), "launches": s2.launches,
+                                               "us_per_iteration": round(s2.kernel_ms * 1e3 / max(1, s2.launches), 3),
+                                               "exact": bool((got == ref).all())}
+            log(json.dumps(r))
+            rows.append(r)
+    wins = [r for r in rows if min(r[k]["ms"] for k in r if k.startswith("relaunch")) < r["persistent"]["ms"]]
+    out = {"metric": "persistent barrier vs relaunch schedule, worklist BFS / SSSP on 1 MI355X", "unit": "ms",
+           "rows": rows, "relaunch_wins": len(wins), "data": "synthetic"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def copy_calibration(args) -> int:
     """Three device-to-device copies of 256 MiB (the runtime's copy kernel):
     the reference reading for scaling FETCH_SIZE / WRITE_SIZE counters."""
@@ -468,10 +505,10 @@ def main() -> int:
                     help="c2: wgs,groups,delta,hops,selector (skips the autotuner; selector 0 none, 4 MLP)")
     ap.add_argument("--engine-only", action="store_true",
                     help="c2 / c3: time the engine's kernels only (profiler runs)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "copy", "probe"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "copy", "probe", "sched"],
                     help="c2 (default): the headline grid BFS + SSSP; c1 colour R-MAT 14; "
                          "c3 MIS + BC R-MAT 22; c4 4096-graph batch; copy: counter calibration; "
-                         "probe: memory latency, protocol and barrier costs")
+                         "probe: memory latency, protocol and barrier costs; sched: persistent vs relaunch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -479,7 +516,7 @@ def main() -> int:
         return 2
     if args.config != "c2":
         return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "copy": copy_calibration,
-                "probe": bench_probe}[args.config](args)
+                "probe": bench_probe, "sched": bench_sched}[args.config](args)
 
     import gpu_discovery_barrier_amd as gdb
     from gpu_discovery_barrier_amd import autotune

@@ -199,6 +199,7 @@ def _declare(lib) -> None:
     }
     optional = {
         "gdb_mis": (I, [P, I, I, P, P, P, P, C.POINTER(Opts), C.POINTER(Stats)]),
+        "gdb_color_maxmin": (I, [P, I, I, P, P, P, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_sssp_pull": (I, [P, I, I, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_bc": (I, [P, I, I, P, P, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_bfs_worklist": (I, [P, I, I, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
@@ -540,6 +541,17 @@ def color(ctx: Context, g: DeviceCSR, node_value: DeviceBuffer, opts=None, out: 
     o = _opts(opts)
     check(lib().gdb_color(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr, node_value.ptr, out.ptr, C.byref(o),
                           C.byref(st)), "gdb_color")
+    return out.to_numpy(), st
+
+
+def color_maxmin(ctx: Context, g: DeviceCSR, node_value: DeviceBuffer, opts=None,
+                 out: DeviceBuffer | None = None):
+    """Max-min colouring (colours 2r-1 / 2r for round r's local maxima / minima)."""
+    out = out or DeviceBuffer.empty(g.n, np.int32)
+    st = Stats()
+    o = _opts(opts)
+    check(lib().gdb_color_maxmin(ctx.h, g.n, g.m, g.row.ptr, g.col.ptr, node_value.ptr, out.ptr, C.byref(o),
+                                 C.byref(st)), "gdb_color_maxmin")
     return out.to_numpy(), st
 
 

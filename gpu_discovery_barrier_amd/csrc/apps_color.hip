@@ -85,6 +85,86 @@ __global__ __launch_bounds__(WGS) void k_color_plain(gdb_kernel_ctx_t* ctx, cons
   }
 }
 
+// Max-min colouring (gdb_color_maxmin): round r gives colour 2r-1 to every
+// uncoloured vertex whose value exceeds all its uncoloured neighbours' and
+// colour 2r to every one below all of them (self-loops ignored).  Phase 1
+// records, per uncoloured vertex, whether it is a local maximum / minimum
+// (a 2-bit code, or 0); phase 2 applies the codes.  Two barriers per round:
+// phase 1 reads colours that phase 2 writes.  Rows of at most kSerialLimit
+// edges are walked by their lane; longer rows by the whole wave, 64 edges
+// per step, with the owner's vertex and value broadcast.
+template <int WGS>
+__global__ __launch_bounds__(WGS) void k_color_maxmin(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
+                                                      const int* __restrict__ col,
+                                                      const float* __restrict__ nv, int* color, int* code,
+                                                      int* stops, int n, int max_rounds, int* out) {
+  DISCOVERY_PROTOCOL(ctx);
+  const int tid = p_get_global_id(ctx, &local_ctx);
+  const int stride = p_get_global_size(ctx, &local_ctx);
+  const int lane = gdb::lane_id();
+  int r = 1, nbar = 0;
+  for (;;) {
+    if (tid == 0) gdb::st_rlx(&stops[(r + 1) % 3], 0);
+    bool any = false;
+    GDB_FOR_UNIFORM(v, valid, tid, stride, n) {
+      const bool act = valid && color[v] == -1;
+      const int b = act ? row[v] : 0, e = act ? row[v + 1] : 0;
+      const float x = act ? nv[v] : 0.f;
+      any |= act;
+      bool is_max = act, is_min = act;
+      const bool heavy = act && (e - b) > kSerialLimit;
+      if (act && !heavy) {
+        for (int k = b; k < e; ++k) {
+          const int u = col[k];
+          if (u == v || (unsigned)u >= (unsigned)n || color[u] != -1) continue;
+          const float y = nv[u];
+          is_max &= x > y;
+          is_min &= x < y;
+        }
+      }
+      unsigned long long mask = __ballot(heavy);
+      while (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        const int lb = gdb::bcast(b, leader), le = gdb::bcast(e, leader), lv = gdb::bcast(v, leader);
+        const float lx = __shfl(x, leader, 64);
+        bool mx = true, mn = true;
+        for (int k = lb + lane; k < le; k += 64) {
+          const int u = col[k];
+          if (u == lv || (unsigned)u >= (unsigned)n || color[u] != -1) continue;
+          const float y = nv[u];
+          mx &= lx > y;
+          mn &= lx < y;
+        }
+        mx = !gdb::wave_any(!mx);
+        mn = !gdb::wave_any(!mn);
+        if (lane == leader) {
+          is_max = mx;
+          is_min = mn;
+        }
+        mask &= mask - 1;
+      }
+      if (act) code[v] = is_max ? 1 : (is_min ? 2 : 0);
+    }
+    if (gdb::wave_any(any) && lane == 0) gdb::st_rlx(&stops[r % 3], 1);
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    if (gdb::ld_rlx(&stops[r % 3]) == 0) break;
+    if (r > max_rounds) {
+      gdb::set_error(ctx, GDB_ERR_QUEUE_OVERFLOW);
+      break;
+    }
+    for (int v = tid; v < n; v += stride)
+      if (color[v] == -1 && code[v] != 0) color[v] = code[v] == 1 ? 2 * r - 1 : 2 * r;
+    if (!discovery_barrier(ctx, &local_ctx)) return;
+    ++nbar;
+    ++r;
+  }
+  if (tid == 0) {
+    out[0] = r - 1;
+    out[1] = nbar;
+  }
+}
+
 // Order-preserving int key of a float (IEEE total order, -0 taken as +0):
 // key(a) > key(b) iff a > b for non-NaN a, b, so max and compares can run on
 // ints and atomicMax combines partial maxima exactly.
@@ -470,6 +550,57 @@ extern "C" int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* 
   }
   rc = check_device_error(ctx);
   if (rc == GDB_E_OVERFLOW) rc = GDB_E_TIMEOUT;  // round cap hit: equal priorities never resolve
+  s.work_edges = m;
+  s.work_items = n;
+  if (stats) *stats = s;
+  return rc;
+}
+
+extern "C" int gdb_color_maxmin(gdb_ctx* ctx, int n, int m, const int* row, const int* col,
+                                const float* node_value, int* color, const gdb_opts* opts, gdb_stats* stats) {
+  if (!ctx || n < 0 || m < 0 || (n > 0 && (!row || !node_value || !color)) || (m > 0 && !col))
+    return GDB_E_INVALID;
+  gdb_opts o{};
+  if (opts) o = *opts;
+  const int wgs = default_wgs(&o);
+  gdb_stats s{};
+  if (n == 0) {
+    if (stats) *stats = s;
+    return GDB_SUCCESS;
+  }
+  const size_t bytes = carve_bytes<int>(n) + carve_bytes<int>(16);
+  int rc = ensure_workspace(ctx, bytes);
+  if (rc) return rc;
+  Carve cv(ctx->ws);
+  int* code = cv.take<int>(n);
+  int* flags = cv.take<int>(16);
+  rc = fill_i32(ctx, color, -1, n);
+  if (!rc) rc = fill_i32(ctx, flags, 0, 16);
+  if (!rc) rc = prepare_launch(ctx);
+  if (rc) return rc;
+  int* out = flags + 8;
+  rc = gdb_dispatch_wgs(wgs, [&](auto W) -> int {
+    constexpr int WG = decltype(W)::value;
+    int groups = 0;
+    int r2 = launch_groups(ctx, (const void*)k_color_maxmin<WG>, &o, n, &groups);
+    if (r2) return r2;
+    r2 = timer_start(ctx);
+    if (r2) return r2;
+    hipLaunchKernelGGL(k_color_maxmin<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col, node_value,
+                       color, code, flags, n, n + 1, out);
+    GDB_HIP(hipGetLastError());
+    return timer_stop(ctx, &s.kernel_ms);
+  });
+  if (rc) return rc;
+  s.launches = 1;
+  int hout[2] = {0, 0};
+  GDB_HIP(hipMemcpy(hout, out, sizeof(hout), hipMemcpyDeviceToHost));
+  s.rounds = hout[0];
+  s.barriers = hout[1];
+  rc = read_participating(ctx, &s.participating);
+  if (rc) return rc;
+  rc = check_device_error(ctx);
+  if (rc == GDB_E_OVERFLOW) rc = GDB_E_TIMEOUT;  // round cap hit: adjacent equal values never resolve
   s.work_edges = m;
   s.work_items = n;
   if (stats) *stats = s;

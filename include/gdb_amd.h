@@ -239,6 +239,16 @@ int gdb_tune_lookup(const char* solver, long long n, gdb_tune_entry* out);
 /* Pannotia colouring (max rule): color[v] = round in which v was coloured. */
 int gdb_color(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const float* node_value,
               int* color, const gdb_opts* opts, gdb_stats* stats);
+/* Max-min colouring: the two-sided variant of the max rule above, which the
+ * reference does not implement (it colours with the max rule only,
+ * color_kernel.cl:229-237).  Round r gives colour 2r-1 to every uncoloured
+ * vertex whose value exceeds all its uncoloured neighbours' and 2r to every
+ * one below all of them (self-loops ignored); a proper colouring for
+ * distinct values, in about half the max rule's rounds.  One persistent
+ * launch, two barriers per round; stats->rounds = rounds.  Adjacent equal
+ * values never resolve: GDB_E_TIMEOUT after n+1 rounds. */
+int gdb_color_maxmin(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const float* node_value,
+                     int* color, const gdb_opts* opts, gdb_stats* stats);
 /* Pannotia MIS: s[v] = 2 for set members, 0 otherwise. */
 int gdb_mis(gdb_ctx* ctx, int n, int m, const int* row, const int* col, const float* node_value,
             int* s, const gdb_opts* opts, gdb_stats* stats);

@@ -36,6 +36,7 @@ def lib():
         sig = {
             "orc_color_ref": (I, [I, P, P, P, P]),
             "orc_color_tb": (I, [I, P, P, P, P]),
+            "orc_color_maxmin_ref": (I, [I, P, P, P, P]),
             "orc_mis_ref": (I, [I, P, P, P, P]),
             "orc_mis_tb": (I, [I, P, P, P, P]),
             "orc_sssp_pull_ref": (I, [I, P, P, P, I, P]),
@@ -85,6 +86,14 @@ def color_tb(row, col, nv):
     n = row.size - 1
     out = np.empty(n, np.int32)
     r = lib().orc_color_tb(n, _p(row), _p(col), _p(nv), _p(out))
+    return out, r
+
+
+def color_maxmin_ref(row, col, nv):
+    row, col, nv = _i32(row), _i32(col), np.ascontiguousarray(nv, np.float32)
+    n = row.size - 1
+    out = np.empty(n, np.int32)
+    r = lib().orc_color_maxmin_ref(n, _p(row), _p(col), _p(nv), _p(out))
     return out, r
 
 

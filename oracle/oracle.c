@@ -61,6 +61,48 @@ int orc_color_ref(int n, const int* row, const int* col, const float* nv, int* c
   return graph_color;
 }
 
+/* Max-min colouring (no reference counterpart: the reference colours with
+ * the max rule only, color_kernel.cl:229-237; the max-min rule is the
+ * two-sided variant of the same Jones-Plassmann-style scheme).  Round
+ * r = 1, 2, ...: among the uncoloured vertices, v takes colour 2r-1 when
+ * its value exceeds every uncoloured neighbour's (self-loops ignored) and
+ * colour 2r when it is below every one of them; decisions use the state at
+ * the start of the round.  Returns the number of rounds that coloured
+ * something (0 for an empty graph), or -1 when n+1 rounds pass with
+ * vertices left (adjacent equal values never resolve). */
+int orc_color_maxmin_ref(int n, const int* row, const int* col, const float* nv, int* color) {
+  for (int i = 0; i < n; ++i) color[i] = -1;
+  int* next = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  if (!next) return -2;
+  int r = 0, left = n;
+  while (left > 0) {
+    ++r;
+    if (r > n + 1) {
+      free(next);
+      return -1;
+    }
+    for (int v = 0; v < n; ++v) {
+      next[v] = color[v];
+      if (color[v] != -1) continue;
+      int is_max = 1, is_min = 1;
+      for (int e = row[v]; e < row[v + 1]; ++e) {
+        const int u = col[e];
+        if (u == v || color[u] != -1) continue;
+        if (!(nv[v] > nv[u])) is_max = 0;
+        if (!(nv[v] < nv[u])) is_min = 0;
+      }
+      if (is_max) next[v] = 2 * r - 1;
+      else if (is_min) next[v] = 2 * r;
+    }
+    for (int v = 0; v < n; ++v) {
+      if (color[v] == -1 && next[v] != -1) --left;
+      color[v] = next[v];
+    }
+  }
+  free(next);
+  return r;
+}
+
 static const float* g_sort_key;
 static int cmp_desc(const void* a, const void* b) {
   const float x = g_sort_key[*(const int*)a], y = g_sort_key[*(const int*)b];

@@ -31,6 +31,12 @@ int orc_color_ref(int n, const int* row, const int* col, const float* nv, int* c
  * (0 when empty); valid for distinct values >= 0. */
 int orc_color_tb(int n, const int* row, const int* col, const float* nv, int* color);
 
+/* Max-min colouring (the two-sided variant; no reference counterpart):
+ * round r gives colour 2r-1 to uncoloured local maxima and 2r to uncoloured
+ * local minima among uncoloured neighbours (self-loops ignored).  Returns
+ * the rounds, or -1 if adjacent equal values livelock it. */
+int orc_color_maxmin_ref(int n, const int* row, const int* col, const float* nv, int* color);
+
 /* MIS (s = 2 in the set, 0 otherwise).  Returns rounds or -1 (livelock). */
 int orc_mis_ref(int n, const int* row, const int* col, const float* nv, int* s);
 /* Greedy sequential MIS in increasing value order (distinct values). */

@@ -110,3 +110,43 @@ def test_color_tuned_blockers(ctx, shape, order):
                   schedule=gdb.SCHED_PERSISTENT, variant=gdb.VARIANT_TUNED)
     assert (got == want).all()
     assert st.rounds == rr
+
+
+# ---------------------------------------------------------------- max-min colouring
+@pytest.mark.parametrize("wgs,groups", [(64, 3), (256, 0), (1024, 16)])
+@pytest.mark.parametrize("graph", ["rmat14", "grid", "gnp", "hubs"])
+def test_color_maxmin_matches_oracle(ctx, graph, wgs, groups):
+    if graph == "rmat14":
+        g = gdb.gen_rmat(14, 16, seed=1)  # the C1 graph
+    elif graph == "grid":
+        g = gdb.gen_grid(120, 90, 0.05, seed=2)
+    elif graph == "gnp":
+        g = gdb.gen_gnp(20000, 8.0, seed=3)
+    else:  # R-MAT with heavy hub rows walked by whole waves
+        g = gdb.gen_rmat(12, 64, seed=4)
+    nv = (gdb.permutation(g.n, 9).astype(np.float64) / (g.n + 1)).astype(np.float32)
+    want, rounds = oracle.color_maxmin_ref(g.row, g.col, nv)
+    dg = gdb.DeviceCSR(g, weights=False)
+    got, st = gdb.color_maxmin(ctx, dg, gdb.DeviceBuffer.from_numpy(nv), gdb.Opts.make(wgs=wgs, groups=groups))
+    print(graph, wgs, groups, st.as_dict())
+    assert (got == want).all()
+    assert st.rounds == rounds
+
+
+def test_color_maxmin_fewer_rounds_than_max_rule(ctx):
+    g = gdb.gen_rmat(14, 16, seed=1)
+    nv = (gdb.permutation(g.n, 1).astype(np.float64) / (g.n + 1)).astype(np.float32)
+    dg = gdb.DeviceCSR(g, weights=False)
+    d_nv = gdb.DeviceBuffer.from_numpy(nv)
+    _, s_mm = gdb.color_maxmin(ctx, dg, d_nv, gdb.Opts.make(wgs=256, groups=64))
+    _, s_max = gdb.color(ctx, dg, d_nv, gdb.Opts.make(wgs=256, groups=64, variant=gdb.VARIANT_TUNED))
+    print("max-min rounds", s_mm.rounds, "max rounds", s_max.rounds)
+    assert s_mm.rounds < s_max.rounds
+
+
+def test_color_maxmin_ties_report_timeout(ctx):
+    row, col, _ = csr_from_edges(4, [(0, 1), (1, 2), (2, 3)])
+    dg = gdb.DeviceCSR(gdb.CSR(4, row, col, np.ones_like(col)), weights=False)
+    with pytest.raises(gdb.GdbError) as e:
+        gdb.color_maxmin(ctx, dg, gdb.DeviceBuffer.from_numpy(np.full(4, 0.5, np.float32)), gdb.Opts.make(wgs=64))
+    assert e.value.code == gdb.GDB_E_TIMEOUT
