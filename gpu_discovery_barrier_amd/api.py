@@ -198,6 +198,8 @@ def _declare(lib) -> None:
         "gdb_csr_transpose": (I, [C.POINTER(HostCSRStruct), C.POINTER(HostCSRStruct)]),
     }
     optional = {
+        "gdb_csr_relabel": (I, [C.POINTER(HostCSRStruct), P, C.POINTER(HostCSRStruct)]),
+        "gdb_order_by_degree": (I, [C.POINTER(HostCSRStruct), P]),
         "gdb_mis": (I, [P, I, I, P, P, P, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_color_maxmin": (I, [P, I, I, P, P, P, P, C.POINTER(Opts), C.POINTER(Stats)]),
         "gdb_sssp_pull": (I, [P, I, I, P, P, P, I, P, C.POINTER(Opts), C.POINTER(Stats)]),
@@ -322,6 +324,21 @@ class CSR:
 
     def degrees(self) -> np.ndarray:
         return np.diff(self.row)
+
+    def relabel(self, new_id: np.ndarray) -> "CSR":
+        """The graph with vertex v renamed new_id[v] (gdb_csr_relabel)."""
+        src = _to_struct(self)
+        ids = np.ascontiguousarray(new_id, np.int32)
+        out = HostCSRStruct()
+        check(lib().gdb_csr_relabel(C.byref(src), ids.ctypes.data_as(P), C.byref(out)), "gdb_csr_relabel")
+        return _from_struct(out)
+
+    def order_by_degree(self) -> np.ndarray:
+        """new_id putting vertices in decreasing degree order (ties by id)."""
+        src = _to_struct(self)
+        ids = np.empty(self.n, np.int32)
+        check(lib().gdb_order_by_degree(C.byref(src), ids.ctypes.data_as(P)), "gdb_order_by_degree")
+        return ids
 
 
 def _from_struct(s: HostCSRStruct) -> CSR:

@@ -185,22 +185,22 @@ def tune_batch_many(ctx, quick=False):
     outs = [[api.DeviceBuffer.empty(g.n, np.uint32), api.DeviceBuffer.empty(g.n, np.uint32),
              api.DeviceBuffer.empty(g.n, np.int32)] for g in gs]
 
-    def run(w, gr, dl, q):
+    def run(w, gr, dl):
         ts = []
         for dg, nv, o in zip(dgs, nvs, outs):
             ts.append(api.make_task(api.TASK_BFS, dg, o[0], 0))
             ts.append(api.make_task(api.TASK_SSSP, dg, o[1], 0, delta=dl))
             ts.append(api.make_task(api.TASK_COLOR, dg, o[2], 0, node_value=nv))
-        return api.batch_solve(ctx, ts, 1, api.Opts.make(wgs=w, groups=gr, lds_queue=q))[1].kernel_ms
+        return api.batch_solve(ctx, ts, 1, api.Opts.make(wgs=w, groups=gr))[1].kernel_ms
 
+    # No local hops here, so the batch takes no LDS queue (lds_queue 0).
     trials = []
     for w, gr in shapes((256, 512), (512, 1024, 2048)):
         for dl in (0, 200):
-            for q in ((2048,) if quick else (1024, 2048)):
-                trials.append((run(w, gr, dl, q), w, gr, dl, q))
-    ms, w, gr, dl, q = best_of(trials)
-    log(f"batch (many graphs): wgs {w} groups {gr} delta {dl} lds_queue {q} {ms:.3f} ms ({len(trials)} shapes)")
-    return row("batch", 65536, w, gr, 1, dl, 0, q), trials
+            trials.append((run(w, gr, dl), w, gr, dl))
+    ms, w, gr, dl = best_of(trials)
+    log(f"batch (many graphs): wgs {w} groups {gr} delta {dl} {ms:.3f} ms ({len(trials)} shapes)")
+    return row("batch", 65536, w, gr, 1, dl, 0, 0), trials
 
 
 def main() -> int:

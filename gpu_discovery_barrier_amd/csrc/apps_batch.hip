@@ -1176,6 +1176,11 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
                     nullptr, a.trow, a.tcol, a.tw};
     scratch_total += (ints + 63) & ~(size_t)63;
   }
+  // Only local hops use the LDS queue: without them it takes no LDS (more
+  // groups fit a CU).
+  bool any_hops = false;
+  for (int t = 0; t < ntasks; ++t) any_hops |= tasks[t].algo != GDB_TASK_COLOR && tasks[t].hops > 0;
+  if (!any_hops) P->qcap = 0;
   P->dyn_lds = sizeof(int) * 2 * kLqFields * (size_t)P->qcap +
                ((cfg.mode == GDB_SEL_MLP && cfg.w) ? sizeof(unsigned short) * gdbsel::kParams : 0);
   int rc = gdb_dispatch_wgs(P->wgs, [&](auto W) -> int {

@@ -271,4 +271,57 @@ int gdb_csr_transpose(const gdb_host_csr* in, gdb_host_csr* out) {
   return GDB_GRAPH_OK;
 }
 
+int gdb_csr_relabel(const gdb_host_csr* in, const int* new_id, gdb_host_csr* out) {
+  if (!in || !new_id || !out || in->n < 0 || in->m < 0) return GDB_GRAPH_E_INVALID;
+  *out = gdb_host_csr{};
+  const int n = in->n, m = in->m;
+  std::vector<int> old_of((size_t)n, -1);
+  for (int v = 0; v < n; ++v) {
+    const int x = new_id[v];
+    if (x < 0 || x >= n || old_of[x] != -1) return GDB_GRAPH_E_INVALID;  // not a permutation
+    old_of[x] = v;
+  }
+  out->n = n;
+  out->m = m;
+  out->row = (int*)calloc((size_t)n + 1, sizeof(int));
+  out->col = (int*)malloc(sizeof(int) * (size_t)(m ? m : 1));
+  out->w = (int*)malloc(sizeof(int) * (size_t)(m ? m : 1));
+  if (!out->row || !out->col || !out->w) return GDB_GRAPH_E_NOMEM;
+  for (int x = 0; x < n; ++x) {
+    const int v = old_of[x];
+    out->row[x + 1] = out->row[x] + (in->row[v + 1] - in->row[v]);
+  }
+  std::vector<std::pair<int, int>> nb;
+  for (int x = 0; x < n; ++x) {
+    const int v = old_of[x];
+    nb.clear();
+    for (int k = in->row[v]; k < in->row[v + 1]; ++k) {
+      const int u = in->col[k];
+      if (u < 0 || u >= n) return GDB_GRAPH_E_INVALID;
+      nb.emplace_back(new_id[u], in->w ? in->w[k] : 1);
+    }
+    std::stable_sort(nb.begin(), nb.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+      return a.first < b.first;
+    });
+    int p = out->row[x];
+    for (const auto& e : nb) {
+      out->col[p] = e.first;
+      out->w[p] = e.second;
+      ++p;
+    }
+  }
+  return GDB_GRAPH_OK;
+}
+
+int gdb_order_by_degree(const gdb_host_csr* g, int* new_id) {
+  if (!g || !new_id || g->n < 0) return GDB_GRAPH_E_INVALID;
+  std::vector<int> order((size_t)g->n);
+  for (int v = 0; v < g->n; ++v) order[v] = v;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return g->row[a + 1] - g->row[a] > g->row[b + 1] - g->row[b];
+  });
+  for (int x = 0; x < g->n; ++x) new_id[order[x]] = x;
+  return GDB_GRAPH_OK;
+}
+
 }  // extern "C"

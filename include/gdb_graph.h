@@ -57,6 +57,15 @@ int gdb_gen_permutation(int n, uint64_t seed, int* out);
 int gdb_gen_uniform_ints(long long count, int lo, int hi, uint64_t seed, int* out);
 /* In-edge CSR: row u of the output lists the sources of edges into u. */
 int gdb_csr_transpose(const gdb_host_csr* in, gdb_host_csr* out);
+/* Relabels a graph for locality: vertex v becomes new_id[v] (a permutation
+ * of 0..n-1); rows are laid out in new-id order and each row's neighbours
+ * sorted by new id.  Per-vertex results of a solve on the relabelled graph
+ * map back as result_old[v] = result_new[new_id[v]]. */
+int gdb_csr_relabel(const gdb_host_csr* in, const int* new_id, gdb_host_csr* out);
+/* new_id ordering vertices by decreasing degree (ties by id): the hubs of a
+ * power-law graph, which every traversal touches, share the first lines of
+ * every per-vertex array. */
+int gdb_order_by_degree(const gdb_host_csr* g, int* new_id);
 /* GDB_GRAPH_OK if offsets are monotone from 0 to m and every destination is
  * in [0, n). */
 int gdb_csr_validate(const gdb_host_csr* g);

@@ -125,3 +125,27 @@ def test_committed_tuning_table_is_well_formed():
     for r in rows:
         assert r["wgs"] in (64, 128, 256, 512, 1024)
         assert r["lds_queue"] == 0 or 256 <= r["lds_queue"] <= 4096
+
+
+def test_relabel_by_degree_is_an_isomorphism():
+    import numpy as np
+
+    g = gdb.gen_rmat(10, 16, seed=3, wlo=1, whi=50)
+    nid = g.order_by_degree()
+    assert sorted(nid.tolist()) == list(range(g.n))
+    deg = g.degrees()
+    assert (np.diff(deg[np.argsort(nid)]) <= 0).all()  # new ids in decreasing degree
+    r = g.relabel(nid)
+    assert r.m == g.m
+    # Every edge (u, v, w) appears as (new_id[u], new_id[v], w), rows sorted.
+    src = np.repeat(np.arange(g.n), deg)
+    want = sorted(zip(nid[src].tolist(), nid[g.col].tolist(), g.w.tolist()))
+    rsrc = np.repeat(np.arange(r.n), r.degrees())
+    assert sorted(zip(rsrc.tolist(), r.col.tolist(), r.w.tolist())) == want
+    for x in range(0, r.n, 37):
+        seg = r.col[r.row[x]:r.row[x + 1]]
+        assert (seg[1:] >= seg[:-1]).all()
+    import pytest
+
+    with pytest.raises(gdb.GdbError):
+        g.relabel(np.zeros(g.n, np.int32))  # not a permutation
