@@ -92,13 +92,9 @@ __device__ __forceinline__ bool mlp_reuse(const IterState& S, int nin) {
          (long long)nin * 4 <= (long long)S.eval_nin * 5 && (long long)nin * 5 >= (long long)S.eval_nin * 4;
 }
 
-// LDS queue capacity per buffer (entries): gdb_opts.lds_queue, in dynamic
-// LDS.  An entry is a vertex and its row bounds (kLqFields ints, one array
-// per field), so a local step reads the vertex's out-edges (and, in the same
-// round trip, its distance) without first loading its row offsets.
+// LDS queue capacity per buffer (ints): gdb_opts.lds_queue, in dynamic LDS.
 constexpr int kLocalQDefault = 2048;
 constexpr int kLocalQMin = 256, kLocalQMax = 4096;
-constexpr int kLqFields = 3;
 constexpr int kMaxHops = 4096;  // local-step budget limit per team iteration
 
 struct Team {
@@ -107,7 +103,7 @@ struct Team {
   int* ts;        // this team's state lines
   int* gen_cache; // LDS
   int G, rank;
-  int* lq;             // LDS: two local queue buffers of qcap entries each
+  int* lq;             // LDS: two local queue buffers of qcap ints each
   int qcap;
   int* lcnt;           // LDS: three rotating local queue counters
   __device__ int word_off(int w) const { return w * kLine; }
@@ -395,12 +391,6 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
   int* q[2] = {WEIGHTED ? base + 2 * (size_t)n : base, WEIGHTED ? base + 3 * (size_t)n : base + n};
   int* far[3] = {base + 4 * (size_t)n, base + 5 * (size_t)n, base + 6 * (size_t)n};
   const bool near_far = WEIGHTED && !UNIT && k.delta > 0;
-  // LDS queue entries carry row bounds for SSSP, where a local step then
-  // saves a round trip (3.6 instead of 4.2 us on a 2048^2 grid).  A unit
-  // BFS relaxes to many vertices that are already reached, and the row
-  // loads for them cost more than the round trip saved (4.6 instead of
-  // 3.5 us per step).
-  constexpr bool kRowsInQueue = !UNIT;
   const int hops = WEIGHTED ? k.hops : 0;
   // The selector runs on level-synchronous tasks only (no local hops).  The
   // SEL = false instantiation carries none of its code (fewer registers).
@@ -564,12 +554,12 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
       // queue.  The slots are branch-free, so every load is issued before
       // any atomic and every atomic before any compare: the slots cost one
       // chain of memory round trips, not kChunk chains.  An unused slot
-      // loads edge 0 (present whenever a slot is used) and its atomic is
-      // masked off: the atomic unit, not latency, bounds a relaxation step,
-      // so no-op atomics would cost throughput.
+      // loads edge 0 (present whenever a slot is used) and sends a no-op
+      // atomic (min with all ones, max with INT_MIN) to this thread's spare
+      // word, a valid vertex that spreads such atomics over many lines.
+      const int spare = (int)(((unsigned)blockIdx.x * blockDim.x + threadIdx.x) % (unsigned)n);
       auto relax_n = [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], unsigned src_d,
-                         int (&dst)[kChunk], int (&cls)[kChunk], int (&rb)[kChunk], int (&re)[kChunk],
-                         unsigned (&cand)[kChunk]) {
+                         int (&dst)[kChunk], int (&cls)[kChunk]) {
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
           dst[j] = -1;
@@ -583,6 +573,7 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
         if (!any) return;
         unsigned wt[kChunk];
         bool act[kChunk];
+        unsigned cand[kChunk];
         int at[kChunk];
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
@@ -594,15 +585,16 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
         for (int j = 0; j < kChunk; ++j) {
           act[j] = ok[j] && (unsigned)dst[j] < (unsigned)n && wt[j] < kInf;
           if (!ok[j]) dst[j] = -1;
-          at[j] = act[j] ? dst[j] : 0;
+          at[j] = act[j] ? dst[j] : spare;
           cand[j] = src_d + wt[j];
           cls[j] = 0;
         }
         if constexpr (!WEIGHTED) {
+          // An unused slot compares against all ones, which no distance is.
           unsigned old[kChunk];
 #pragma unroll
           for (int j = 0; j < kChunk; ++j)
-            old[j] = act[j] ? gdb::atomic_cas_g(&dist[at[j]], kInf, src_d) : 0u;
+            old[j] = gdb::atomic_cas_g(&dist[at[j]], act[j] ? kInf : ~0u, act[j] ? src_d : ~0u);
 #pragma unroll
           for (int j = 0; j < kChunk; ++j) cls[j] = (act[j] && old[j] == kInf) ? 1 : 0;
         } else {
@@ -611,25 +603,14 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
             // The mark word stays 0 with local hops, so a 32-bit atomic on
             // the distance word does (a 64-bit one has a third of the
             // throughput).
-            // With kRowsInQueue a near candidate, which may become a local
-            // queue entry, has its row bounds loaded beside the atomic, in
-            // the same round trip.
             unsigned old[kChunk];
 #pragma unroll
             for (int j = 0; j < kChunk; ++j)
-              old[j] = act[j] ? gdb::atomic_min_g(reinterpret_cast<unsigned*>(&d64[at[j]]) + 1, cand[j]) : ~0u;
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j) {
-              nearp[j] = cand[j] < thresh;
-              rb[j] = re[j] = 0;
-              if (kRowsInQueue && act[j] && nearp[j]) {
-                rb[j] = gdb::ldg(k.row + at[j]);
-                re[j] = gdb::ldg(k.row + at[j] + 1);
-              }
-            }
+              old[j] = gdb::atomic_min_g(reinterpret_cast<unsigned*>(&d64[at[j]]) + 1, act[j] ? cand[j] : ~0u);
 #pragma unroll
             for (int j = 0; j < kChunk; ++j) {
               imp[j] = act[j] && cand[j] < old[j];
+              nearp[j] = cand[j] < thresh;
               if (imp[j] && nearp[j]) cls[j] = 3;
             }
           } else {
@@ -639,7 +620,7 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
               nearp[j] = cand[j] < thresh;
               const unsigned long long nw =
                   ((unsigned long long)cand[j] << 32) | (nearp[j] ? near_mark : far_mark);
-              old[j] = act[j] ? gdb::atomic_min_g(&d64[at[j]], nw) : ~0ull;
+              old[j] = gdb::atomic_min_g(&d64[at[j]], act[j] ? nw : ~0ull);
             }
 #pragma unroll
             for (int j = 0; j < kChunk; ++j) {
@@ -658,7 +639,7 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
 #pragma unroll
             for (int j = 0; j < kChunk; ++j) {
               const bool f = imp[j] && !nearp[j];
-              fs[j] = f ? gdb::atomic_max_g(&fstamp[dst[j]], gen) : INT_MAX;
+              fs[j] = gdb::atomic_max_g(&fstamp[f ? dst[j] : spare], f ? gen : INT_MIN);
             }
 #pragma unroll
             for (int j = 0; j < kChunk; ++j)
@@ -668,8 +649,7 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
       };
       // Pushes the results of one relaxation step of this wave.  Class 3
       // goes to LDS buffer `lb` when there is room, else to the team queue.
-      auto push_n = [&](const int (&dst)[kChunk], const int (&cls)[kChunk], const int (&rb)[kChunk],
-                        const int (&re)[kChunk], int lb, int* lc) {
+      auto push_n = [&](const int (&dst)[kChunk], const int (&cls)[kChunk], int lb, int* lc) {
         bool to_team[kChunk], to_far[kChunk];
         unsigned long long m3[kChunk];
         int n3 = 0;
@@ -689,16 +669,8 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
           for (int j = 0; j < kChunk; ++j) {
             if (cls[j] == 3) {
               const int at = pos + __popcll(m3[j] & lt);
-              if (at < qcap) {
-                int* const e = lq + lb * kLqFields * qcap + at;
-                e[0] = dst[j];
-                if (kRowsInQueue) {
-                  e[qcap] = rb[j];
-                  e[2 * qcap] = re[j];
-                }
-              } else {
-                to_team[j] = atomicExch(&gstamp[dst[j]], it) != it;
-              }
+              if (at < qcap) lq[lb * qcap + at] = dst[j];
+              else to_team[j] = atomicExch(&gstamp[dst[j]], it) != it;
             }
             pos += __popcll(m3[j]);
           }
@@ -706,36 +678,21 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
         gdb::wave_push_n<kChunk>(to_team, dst, outq, out_tail);
         if (near_far) gdb::wave_push_n<kChunk>(to_far, dst, fpile, ftail);
       };
-      // Relaxes the out-edges of items[start, count) with stride `step`:
-      // team queue vertices (src_lds null) or the entries of an LDS queue
-      // buffer, which carry their row bounds when kRowsInQueue.
-      auto process = [&](const int* src_items, const int* src_lds, int count, int start, int step, int lb,
-                         int* lc) {
+      // Relaxes the out-edges of items[start, count) with stride `step`.
+      auto process = [&](const int* src_items, int count, int start, int step, int lb, int* lc) {
         GDB_FOR_UNIFORM(i, valid, start, step, count) {
           int v = -1, b = 0, e = 0;
           unsigned dv = 0;
-          if (valid && src_lds) {
-            v = src_lds[i];
-            if (kRowsInQueue) {
-              b = src_lds[qcap + i];
-              e = src_lds[2 * qcap + i];
-            } else {
-              b = k.row[v];
-              e = k.row[v + 1];
-            }
-            dv = (unsigned)(d64[v] >> 32);
-            ++items;
-            edges += e - b;
-          } else if (valid) {
+          if (valid) {
             v = src_items[i];
             if ((unsigned)v >= (unsigned)n) valid = 0;
-            if (valid) {
-              b = k.row[v];
-              e = k.row[v + 1];
-              dv = WEIGHTED ? (unsigned)(d64[v] >> 32) : (unsigned)it;
-              ++items;
-              edges += e - b;
-            }
+          }
+          if (valid) {
+            b = k.row[v];
+            e = k.row[v + 1];
+            dv = WEIGHTED ? (unsigned)(d64[v] >> 32) : (unsigned)it;
+            ++items;
+            edges += e - b;
           }
           // Light rows: each lane relaxes its own edges four at a time, the
           // four atomics of a chunk issued back to back.
@@ -744,37 +701,35 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
           const int maxlen = gdb::wave_max(len);
           for (int c = 0; c < maxlen; c += kChunk) {
             bool ok[kChunk];
-            int kk[kChunk], dst[kChunk], cls[kChunk], rb[kChunk], re[kChunk];
-            unsigned cand[kChunk];
+            int kk[kChunk], dst[kChunk], cls[kChunk];
 #pragma unroll
             for (int j = 0; j < kChunk; ++j) {
               ok[j] = c + j < len;
               kk[j] = b + c + j;
             }
-            relax_n(ok, kk, dv, dst, cls, rb, re, cand);
-            push_n(dst, cls, rb, re, lb, lc);
+            relax_n(ok, kk, dv, dst, cls);
+            push_n(dst, cls, lb, lc);
           }
           // Heavy rows: the whole wave walks one row, 64 edges per step.
           gdb::wave_coop_foreach(
               heavy, b, e, dv,
               [&](int e, bool ok, unsigned src_d) {
                 bool oks[kChunk];
-                int kk[kChunk], dst[kChunk], cls[kChunk], rb[kChunk], re[kChunk];
-                unsigned cand[kChunk];
+                int kk[kChunk], dst[kChunk], cls[kChunk];
 #pragma unroll
                 for (int j = 0; j < kChunk; ++j) {
                   oks[j] = j == 0 && ok;
                   kk[j] = e;
                 }
-                relax_n(oks, kk, src_d, dst, cls, rb, re, cand);
-                push_n(dst, cls, rb, re, lb, lc);
+                relax_n(oks, kk, src_d, dst, cls);
+                push_n(dst, cls, lb, lc);
               },
               -1);
         }
       };
       // The team's share of the iteration; improved vertices go to LDS
       // buffer 1 (local step 1 reads it).
-      process(in, nullptr, nin, tid, stride, 1, &lcnt[1]);
+      process(in, nin, tid, stride, 1, &lcnt[1]);
       // Local steps: this group alone relaxes what it just improved.  Only
       // LDS is exchanged between its waves here (the relaxations are
       // returning atomics; queue stores are read after the team barrier,
@@ -785,8 +740,7 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
         const int ln = min(lcnt[s % 3], qcap);
         if (ln == 0) break;
         if (threadIdx.x == 0) lcnt[(s + 2) % 3] = 0;
-        process(nullptr, lq + (s & 1) * kLqFields * qcap, ln, (int)threadIdx.x, (int)blockDim.x, (s + 1) & 1,
-                &lcnt[(s + 1) % 3]);
+        process(lq + (s & 1) * qcap, ln, (int)threadIdx.x, (int)blockDim.x, (s + 1) & 1, &lcnt[(s + 1) % 3]);
       }
       if (hops > 0 && s > hops) {
         // Budget spent: flush the last local buffer to the team queue.
@@ -796,7 +750,7 @@ __device__ __forceinline__ int team_worklist(Team& T, const DevTask& k, unsigned
           int v = 0;
           bool p = false;
           if (valid) {
-            v = lq[(s & 1) * kLqFields * qcap + i];
+            v = lq[(s & 1) * qcap + i];
             p = atomicExch(&gstamp[v], it) != it;
           }
           gdb::wave_push(p, v, outq, out_tail);
@@ -1043,11 +997,11 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(WGS == 512 
   DISCOVERY_PROTOCOL(ctx);
   __shared__ int gen_cache;
   __shared__ int lcnt[3];
-  // Dynamic LDS: the two local queue buffers (qcap entries each), then the
-  // MLP weights (MLP mode only).
+  // Dynamic LDS: the two local queue buffers (qcap ints each), then the MLP
+  // weights (MLP mode only).
   extern __shared__ int dyn_lds[];
   int* const lq = dyn_lds;
-  unsigned short* const sel_w = (unsigned short*)(dyn_lds + 2 * kLqFields * qcap);
+  unsigned short* const sel_w = (unsigned short*)(dyn_lds + 2 * qcap);
   const int P = p_get_num_groups(ctx, &local_ctx);
   const int id = p_get_group_id(ctx, &local_ctx);
   if (G > P) G = P;
@@ -1181,14 +1135,11 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
   bool any_hops = false;
   for (int t = 0; t < ntasks; ++t) any_hops |= tasks[t].algo != GDB_TASK_COLOR && tasks[t].hops > 0;
   if (!any_hops) P->qcap = 0;
-  P->dyn_lds = sizeof(int) * 2 * kLqFields * (size_t)P->qcap +
+  P->dyn_lds = sizeof(int) * 2 * (size_t)P->qcap +
                ((cfg.mode == GDB_SEL_MLP && cfg.w) ? sizeof(unsigned short) * gdbsel::kParams : 0);
   int rc = gdb_dispatch_wgs(P->wgs, [&](auto W) -> int {
     constexpr int WG = decltype(W)::value;
     const void* kern = cfg.mode != GDB_SEL_NONE ? (const void*)k_batch<WG, true> : (const void*)k_batch<WG, false>;
-    // Above 64 KiB of dynamic LDS a kernel has to opt in (a CU has 160 KiB).
-    if (P->dyn_lds > 65536)
-      GDB_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->dyn_lds));
     return launch_groups(ctx, kern, &o, (long long)GDB_MAX_GROUPS * WG, &P->groups, P->dyn_lds);
   });
   if (rc) return rc;

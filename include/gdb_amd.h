@@ -88,7 +88,7 @@ typedef struct {
   int variant;        /* GDB_VARIANT_*                                         */
   int barrier;        /* GDB_BARRIER_* or -1 for the context default           */
   int relaunch_check; /* relaunch schedule: rounds between host stop checks (0/1 = every round) */
-  int lds_queue;      /* batched engine: entries (12 B) per LDS queue buffer of a group's local hops,
+  int lds_queue;      /* batched engine: ints per LDS queue buffer of a group's local hops,
                          256..4096 (0 = the tuning table's value, else 2048) */
 } gdb_opts;
 
