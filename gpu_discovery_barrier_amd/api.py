@@ -105,6 +105,7 @@ class Task(C.Structure):
         ("out", C.c_void_p),
         ("delta", C.c_int),
         ("hops", C.c_int),
+        ("owned", C.c_int),
         ("trow", C.c_void_p),
         ("tcol", C.c_void_p),
         ("tw", C.c_void_p),
@@ -645,11 +646,13 @@ def verify_distances(ctx: Context, g: DeviceCSR, dist: DeviceBuffer, weighted: b
 # ---------------------------------------------------------------------------
 def make_task(algo: int, g: "DeviceCSR", out: DeviceBuffer, source: int = 0,
               node_value: DeviceBuffer | None = None, delta: int = 0, hops: int = 0,
-              transpose: "DeviceCSR | None" = None) -> Task:
+              transpose: "DeviceCSR | None" = None, owned: bool = False) -> Task:
     """`transpose` is the in-edge CSR (the graph itself when it is symmetric);
-    it enables the pull arms of the schedule selector.  The task holds
-    references to the device objects it points into, so they stay allocated
-    while the task is alive."""
+    it enables the pull arms of the schedule selector.  `owned` asks for the
+    owner-partitioned solve (each group of the team keeps a contiguous vertex
+    range in LDS; see gdb_task.owned).  The task holds references to the
+    device objects it points into, so they stay allocated while the task is
+    alive."""
     t = Task()
     t._keep = (g, out, node_value, transpose)
     t.algo, t.n, t.m = algo, g.n, g.m
@@ -660,6 +663,7 @@ def make_task(algo: int, g: "DeviceCSR", out: DeviceBuffer, source: int = 0,
     t.out = out.ptr
     t.delta = delta
     t.hops = hops
+    t.owned = 1 if owned else 0
     if transpose is not None:
         if transpose.n != g.n or transpose.m != g.m:
             raise ValueError("transpose must have the graph's n and m")

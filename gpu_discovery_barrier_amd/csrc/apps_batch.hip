@@ -39,7 +39,11 @@ constexpr int kLine = GDB_LINE_BYTES / 4;  // ints per 128 B line
 // Team state, one 128 B line per word.  TW_RESUME and TW_SEL are lines of
 // several words (see below).
 enum TeamWord { TW_COUNT = 0, TW_GEN, TW_TASK, TW_TAIL0, TW_TAIL1, TW_TAIL2, TW_FAR0, TW_FAR1, TW_FAR2,
-                TW_STOP0, TW_STOP1, TW_STOP2, TW_PARK, TW_RESUME, TW_SEL, TW_WORDS };
+                TW_STOP0, TW_STOP1, TW_STOP2, TW_PARK, TW_RESUME, TW_SEL,
+                TW_OARR0, TW_OARR1, TW_OARR2, TW_OFIN0, TW_OFIN1, TW_OFIN2, TW_OMIN0, TW_OMIN1, TW_OMIN2,
+                TW_WORDS };
+// TW_OARR* / TW_OFIN* / TW_OMIN*: arrival count, report count and smallest
+// deferred distance of the owned solve's epochs (three, rotating).
 // TW_RESUME line: what a parked group needs to rejoin (written before the
 // release of TW_PARK).
 enum ResumeWord {
@@ -51,7 +55,7 @@ constexpr int kNinBits = 30;  // barrier payload = frontier size | arm << 30
 constexpr int kNinMask = (1 << kNinBits) - 1;
 
 struct DevTask {
-  int algo, n, m, source, delta, hops;
+  int algo, n, m, source, delta, hops, owned;
   const int* row;
   const int* col;
   const int* w;
@@ -986,22 +990,480 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
   return r;
 }
 
+// ---- Owner-partitioned BFS / SSSP on a team (gdb_task.owned) --------------
+// Group r of a team of G owns the vertex range [r*B, r*B + B), B = ceil(n/G)
+// rounded up to 32, and keeps the range's distances, row offsets, queue
+// marks and queues in LDS.  A relaxation whose destination lies in the range
+// is an LDS atomic, and the group's rounds are separated by workgroup
+// barriers only, so a hop along a path inside a range costs one memory round
+// trip (the edge loads) instead of three.  A relaxation into another range is
+// a fire-and-forget device-scope atomic min on the destination's output
+// word; an owner reads the output words of its boundary vertices (the ones
+// with an in-edge from another range, marked once per task) every few
+// rounds and while it is idle.  (A per-owner counter of such updates, to
+// skip scans that find nothing, measured slower on a 2048^2 grid: every
+// scan there finds something, and the extra atomic per edge costs.)
+//
+// Distance bands: the team relaxes only below a common threshold T (SSSP:
+// delta; BFS: kOwnedBfsBand levels); an improvement at or above T waits in
+// the owner's far set.  Without the common band, ranges that run ahead
+// spread distances that later prove too long and relax them again (an
+// order of magnitude more work on a grid).
+//
+// Epochs e = 0, 1, ...: a group with nothing left below T drains its
+// atomics and arrives on the epoch's counter, then keeps polling its
+// boundary; an improvement below T found while fewer than G have arrived
+// un-arrives it (compare-and-swap, so the count never drops from G).  Once
+// all G have arrived no relaxation is in flight: every group scans its
+// boundary once more and reports whether anything below T improved, and the
+// smallest distance in its far set.  If something improved the band stays;
+// otherwise T moves to that smallest distance plus the band width, or the
+// solve ends when every far set is empty.  Every order of relaxations
+// reaches the same fixpoint, so outputs are the exact levels / distances of
+// the other engines.
+//
+// LDS (dynamic, `region` bytes): dist[B] u32 | inq[B/32] | far[B/32] |
+// queue u16[2][qcap] | row offsets[B + 1] when they fit | boundary list u16.
+constexpr int kOwnedScanEvery = 2;  // rounds between boundary scans of a busy group
+constexpr int kOwnedBfsBand = 128;  // BFS levels per band
+// Dynamic LDS of a batch with an owned task: a CU's 160 KiB less the kernel's
+// static LDS (a few hundred bytes).
+constexpr long long kOwnedLdsTotal = 163840 - 1024;
+
+// Bytes of LDS the owned solve needs without the optional parts.
+__host__ __device__ constexpr long long owned_lds_core(long long B, int qcap) {
+  return 4 * B + 2 * (B / 32) * 4 + 4LL * qcap;
+}
+__host__ __device__ inline int owned_range(int n, int G) { return (((n + G - 1) / G) + 31) & ~31; }
+
+// LDS pointers of the owned solve carry their address space, so that its
+// accesses are ds_* instructions (a generic pointer would give flat ones,
+// whose waits also cover the memory loads in flight).
+typedef __attribute__((address_space(3))) unsigned lds_u32;
+typedef __attribute__((address_space(3))) unsigned short lds_u16;
+typedef __attribute__((address_space(3))) int lds_i32;
+typedef __attribute__((address_space(3))) char lds_char;
+typedef __attribute__((address_space(1))) unsigned g_u32;
+typedef __attribute__((address_space(1))) const int g_ci32;
+__device__ __forceinline__ unsigned lds_or(lds_u32* p, unsigned v) {
+  return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ unsigned lds_and(lds_u32* p, unsigned v) {
+  return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <bool UNIT>
+__device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, int region) {
+  __shared__ int s_cnt[3];   // rotating local queue counters
+  __shared__ int s_nbl;      // boundary list length
+  __shared__ int s_flag;     // an improvement was found (scan) / far set non-empty
+  __shared__ unsigned s_min;
+  __shared__ int s_c;
+  __shared__ unsigned s_gmin;
+  const int n = k.n, G = T.G, r = T.rank;
+  const int B = owned_range(n, G);
+  const int lo = min(n, r * B), hi = min(n, lo + B), nb = hi - lo;
+  const int W = B >> 5;
+  const int qc = T.qcap;
+  lds_char* const lb = (lds_char*)T.lq;
+  lds_u32* const dist = (lds_u32*)lb;
+  lds_u32* const inq = dist + B;
+  lds_u32* const farb = inq + W;
+  lds_u16* const q = (lds_u16*)(farb + W);
+  long long used = owned_lds_core(B, qc);
+  lds_i32* rowl = nullptr;
+  if (used + 4LL * (B + 1) <= region) {
+    rowl = (lds_i32*)(lb + used);
+    used += 4LL * (B + 1);
+  }
+  lds_u16* const bl = (lds_u16*)(lb + used);
+  const int blcap = (int)((region - used) / 2);
+  // Device memory through global-address-space pointers (global_*, not
+  // flat_* instructions, whose waits would also cover the LDS operations);
+  // task fields in registers, since k lives in memory these stores could
+  // alias.
+  g_u32* const out = (g_u32*)k.out;
+  g_u32* const bmark = (g_u32*)k.scratch;
+  g_ci32* const row = (g_ci32*)k.row;
+  g_ci32* const col = (g_ci32*)k.col;
+  g_ci32* const wgt = (g_ci32*)k.w;
+  const int delta = k.delta, source = k.source;
+  const int nw = (n + 31) >> 5;
+  const int lane = gdb::lane_id();
+  const unsigned long long lt = (1ull << lane) - 1ull;
+
+  // Team-wide set-up: output words (remote minima land there), boundary
+  // marks and the termination counters.
+  {
+    const int tid = T.tid(), stride = T.stride();
+    for (int v = tid; v < n; v += stride) out[v] = kInf;
+    for (int w = tid; w < nw; w += stride) bmark[w] = 0;
+    if (tid == 0)
+      for (int e = 0; e < 3; ++e) {
+        gdb::st_rlx(T.word(TW_OARR0 + e), 0);
+        gdb::st_rlx(T.word(TW_OFIN0 + e), 0);
+        gdb::st_rlx(T.word(TW_OMIN0 + e), (int)kInf);
+      }
+  }
+  for (int i = threadIdx.x; i < B; i += blockDim.x) dist[i] = kInf;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) {
+    inq[i] = 0;
+    farb[i] = 0;
+  }
+  if (rowl)
+    for (int i = threadIdx.x; i <= nb; i += blockDim.x) rowl[i] = row[lo + i];
+  if (threadIdx.x == 0) {
+    s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
+    s_nbl = 0;
+  }
+  if (!team_barrier(T)) return -1;
+  // Boundary marks: every edge of this range into another range marks its
+  // destination.
+  GDB_FOR_UNIFORM(i, valid, (int)threadIdx.x, (int)blockDim.x, nb) {
+    int b = 0, e = 0;
+    if (valid) {
+      b = row[lo + i];
+      e = row[lo + i + 1];
+    }
+    gdb::wave_coop_foreach(
+        valid, b, e, 0u,
+        [&](int j, bool ok, unsigned) {
+          const int v = ok ? col[j] : lo;
+          if (ok && (unsigned)v < (unsigned)n && (unsigned)(v - lo) >= (unsigned)nb)
+            __hip_atomic_fetch_or(&bmark[v >> 5], 1u << (v & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        },
+        kSerialLimit);
+  }
+  if ((unsigned)(source - lo) < (unsigned)nb && threadIdx.x == 0) {
+    const int s0 = source - lo;
+    dist[s0] = 0;
+    inq[s0 >> 5] |= 1u << (s0 & 31);
+    q[0] = (unsigned short)s0;
+    s_cnt[0] = 1;
+  }
+  if (!team_barrier(T)) return -1;
+  // Boundary list of this range.
+  for (int w = threadIdx.x; w < W; w += blockDim.x) {
+    unsigned bits = (lo + 32 * w < hi) ? __hip_atomic_load(&bmark[(lo >> 5) + w], __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT)
+                                       : 0u;
+    if (bits) {
+      int pos = atomicAdd(&s_nbl, __popc(bits));
+      while (bits) {
+        const int j = __ffs(bits) - 1;
+        bits &= bits - 1;
+        if (pos < blcap) bl[pos] = (unsigned short)(32 * w + j);
+        ++pos;
+      }
+    }
+  }
+  __syncthreads();
+  const int nbl = s_nbl;
+  const bool scan_all = nbl > blcap;  // list did not fit: scan the whole range
+  const int nscan = scan_all ? nb : nbl;
+
+  const unsigned band = UNIT ? (unsigned)kOwnedBfsBand : delta > 0 ? (unsigned)delta : kInf;
+  unsigned thresh = band;
+  long long items = 0, edges = 0;
+  // Relaxes local vertex li to cand on lanes with act.  An improvement below
+  // the threshold joins queue buffer qb (once: its inq bit); one above it, or
+  // one that finds the buffer full, joins the far set.  All lanes of the wave
+  // call it.  Returns whether this lane improved.
+  auto relax_local = [&](bool act, int li, unsigned cand, int qb, int cb) -> bool {
+    bool imp = false, push = false;
+    const unsigned bit = 1u << (li & 31);
+    if (act) {
+      imp = cand < __hip_atomic_fetch_min(&dist[li], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (imp) {
+        if (cand < thresh) push = !(lds_or(&inq[li >> 5], bit) & bit);
+        else lds_or(&farb[li >> 5], bit);
+      }
+    }
+    const unsigned long long m = __ballot(push);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&s_cnt[cb], __popcll(m));
+      base = gdb::bcast(base, leader);
+      if (push) {
+        const int pos = base + __popcll(m & lt);
+        if (pos < qc) {
+          q[qb * qc + pos] = (unsigned short)li;
+        } else {
+          lds_and(&inq[li >> 5], ~bit);
+          lds_or(&farb[li >> 5], bit);
+        }
+      }
+    }
+    return imp;
+  };
+  // Relaxes every boundary vertex from its output word, kScanBatch loads per
+  // thread in flight; sets s_flag when an improvement joined the queue
+  // (below the threshold).
+  constexpr int kScanBatch = 16;
+  auto scan = [&](int qb, int cb) {
+    bool any = false;
+    const int step = kScanBatch * (int)blockDim.x;
+    for (int base = 0; base < nscan; base += step) {
+      int li[kScanBatch];
+      unsigned x[kScanBatch];
+#pragma unroll
+      for (int j = 0; j < kScanBatch; ++j) {
+        const int i = base + j * (int)blockDim.x + (int)threadIdx.x;
+        li[j] = i < nscan ? (scan_all ? i : (int)bl[i]) : -1;
+        x[j] = li[j] >= 0 ? __hip_atomic_load(&out[lo + li[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          : kInf;
+      }
+#pragma unroll
+      for (int j = 0; j < kScanBatch; ++j)
+        any |= relax_local(x[j] < kInf, li[j] < 0 ? 0 : li[j], x[j], qb, cb) && x[j] < thresh;
+    }
+    if (__ballot(any) && lane == 0) s_flag = 1;
+  };
+  // Relaxes the out-edges of queue buffer (qa, count ca) into (qb, cb).
+  auto round = [&](int qa, int ca, int qb, int cb) {
+    const int ln = min(s_cnt[ca], qc);
+    GDB_FOR_UNIFORM(i, valid, (int)threadIdx.x, (int)blockDim.x, ln) {
+      int li = 0, b = 0, e = 0;
+      unsigned du = kInf;
+      if (valid) {
+        li = q[qa * qc + i];
+        // Clear the mark before reading the distance: a later improvement
+        // queues the vertex again.
+        // (A wave's LDS operations complete in order; the signal fence
+        // keeps the compiler from swapping the two.)
+        lds_and(&inq[li >> 5], ~(1u << (li & 31)));
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        du = __hip_atomic_load(&dist[li], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        b = rowl ? rowl[li] : row[lo + li];
+        e = rowl ? rowl[li + 1] : row[lo + li + 1];
+        ++items;
+        edges += e - b;
+      }
+      auto relax_edges = [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], unsigned src_d) {
+        int v[kChunk];
+        unsigned wt[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int ee = ok[j] ? kk[j] : 0;
+          v[j] = ok[j] ? col[ee] : -1;
+          wt[j] = UNIT ? 1u : (ok[j] ? (unsigned)wgt[ee] : kInf);
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const bool loc = (unsigned)(v[j] - lo) < (unsigned)nb;
+          const bool rem = !loc && (unsigned)v[j] < (unsigned)n;
+          const unsigned cand = src_d + wt[j];
+          if (ok[j] && rem && wt[j] < kInf)
+            __hip_atomic_fetch_min(&out[v[j]], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          relax_local(ok[j] && loc && wt[j] < kInf, loc ? v[j] - lo : 0, cand, qb, cb);
+        }
+      };
+      const bool heavy = valid && (e - b) > kSerialLimit;
+      const int len = (valid && !heavy) ? (e - b) : 0;
+      const int maxlen = gdb::wave_max(len);
+      for (int c = 0; c < maxlen; c += kChunk) {
+        bool ok[kChunk];
+        int kk[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          ok[j] = c + j < len;
+          kk[j] = b + c + j;
+        }
+        relax_edges(ok, kk, du);
+      }
+      gdb::wave_coop_foreach(
+          heavy, b, e, du,
+          [&](int j, bool ok, unsigned src_d) {
+            bool oks[kChunk];
+            int kk[kChunk];
+#pragma unroll
+            for (int t = 0; t < kChunk; ++t) {
+              oks[t] = t == 0 && ok;
+              kk[t] = j;
+            }
+            relax_edges(oks, kk, src_d);
+          },
+          -1);
+    }
+  };
+  // Smallest distance in the far set (kInf when empty), in s_min.
+  auto far_min = [&]() {
+    if (threadIdx.x == 0) s_min = kInf;
+    __syncthreads();
+    for (int w = threadIdx.x; w < W; w += blockDim.x) {
+      unsigned bits = farb[w];
+      while (bits) {
+        const int j = __ffs(bits) - 1;
+        bits &= bits - 1;
+        atomicMin(&s_min, (unsigned)dist[32 * w + j]);
+      }
+    }
+    __syncthreads();
+  };
+  // Far set: the vertices improved at or beyond the threshold, or that found
+  // the queue full.  Queues every one now below the threshold; returns
+  // whether it queued any.
+  auto drain_far = [&](int qb, int cb) -> bool {
+    far_min();
+    if (s_min >= thresh) return false;
+    for (int w = threadIdx.x; w < W; w += blockDim.x) {
+      unsigned bits = farb[w];
+      unsigned keep = bits;
+      while (bits) {
+        const int j = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const int li = 32 * w + j;
+        if (dist[li] < thresh) {
+          keep &= ~(1u << j);
+          const unsigned bit = 1u << j;
+          if (!(lds_or(&inq[w], bit) & bit)) {
+            const int pos = atomicAdd(&s_cnt[cb], 1);
+            if (pos < qc) {
+              q[qb * qc + pos] = (unsigned short)li;
+            } else {
+              lds_and(&inq[w], ~bit);
+              keep |= bit;
+            }
+          }
+        }
+      }
+      farb[w] = keep;
+    }
+    __syncthreads();
+    return s_cnt[cb] > 0;
+  };
+
+  const unsigned long long limit = (unsigned long long)(unsigned)gdb::ld_rlx(&T.ctx->timeout_ticks.v);
+  int s = 0;  // round number: queue buffer s & 1, counter s % 3
+  int epoch = 0;
+  bool done = false;
+  __syncthreads();
+  while (!done) {
+    // Local rounds until the queue runs dry.
+    for (;;) {
+      const int ca = s % 3;
+      if (s_cnt[ca] == 0) break;
+      const int cb = (s + 1) % 3, qa = s & 1, qb = qa ^ 1;
+      if (threadIdx.x == 0) s_cnt[(s + 2) % 3] = 0;
+      round(qa, ca, qb, cb);
+      if (s % kOwnedScanEvery == kOwnedScanEvery - 1) scan(qb, cb);
+      ++s;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) s_cnt[(s + 1) % 3] = 0;
+    if (drain_far(s & 1, s % 3)) continue;
+    // Idle: arrive on this epoch's counter and poll the boundary.
+    int* const arr = T.word(TW_OARR0 + epoch % 3);
+    int* const fin = T.word(TW_OFIN0 + epoch % 3);
+    gdb::drain_stores();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (r == 0) {  // counters of epoch + 1 (last used by epoch - 2, which every group has left)
+        gdb::st_rlx(T.word(TW_OARR0 + (epoch + 1) % 3), 0);
+        gdb::st_rlx(T.word(TW_OFIN0 + (epoch + 1) % 3), 0);
+        gdb::st_rlx(T.word(TW_OMIN0 + (epoch + 1) % 3), (int)kInf);
+      }
+      gdb::fadd(arr, 1);
+      s_flag = 0;
+    }
+    const unsigned long long t0 = gdb::now();
+    bool active = false, pending = false;
+    for (unsigned it = 0;; ++it) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        s_c = gdb::ld_acq(arr);
+        if (s_c < G && (it & 15u) == 15u) {
+          if (gdb::ld_rlx(&T.ctx->error.v) != GDB_OK || gdb::now() - t0 > limit) {
+            gdb::set_error(T.ctx, GDB_ERR_BARRIER_TIMEOUT);
+            T.l->aborted = 1;
+          }
+        }
+      }
+      __syncthreads();
+      if (T.l->aborted) return -1;
+      if (s_c == G) break;
+      scan(s & 1, s % 3);
+      __syncthreads();
+      if (s_flag) {
+        if (threadIdx.x == 0) {
+          int c = gdb::ld_rlx(arr);
+          s_c = c;
+          while (c < G) {
+            if (__hip_atomic_compare_exchange_strong(arr, &c, c - 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+              s_c = -1;  // un-arrived
+              break;
+            }
+          }
+        }
+        __syncthreads();
+        if (s_c == -1) active = true;
+        else pending = true;
+        break;
+      }
+      if (threadIdx.x == 0) __builtin_amdgcn_s_sleep(8);
+    }
+    if (active) continue;
+    // Everyone arrived: final scan, then report (improved below the
+    // threshold?, smallest far distance).
+    if (!pending) {
+      if (threadIdx.x == 0) s_flag = 0;
+      __syncthreads();
+      scan(s & 1, s % 3);
+      __syncthreads();
+      pending = s_flag != 0;
+    }
+    far_min();
+    if (threadIdx.x == 0) {
+      int* const mnw = T.word(TW_OMIN0 + epoch % 3);
+      if (s_min < kInf)
+        __hip_atomic_fetch_min((unsigned*)mnw, s_min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      gdb::fadd(fin, 1 + (pending ? (1 << 16) : 0));
+      int f = 0;
+      const bool ok = gdb::wait_while(
+          T.ctx, [&] { return ((f = gdb::ld_acq(fin)) & 0xffff) < G; }, GDB_ERR_BARRIER_TIMEOUT);
+      if (!ok) T.l->aborted = 1;
+      s_c = f >> 16;
+      s_gmin = (unsigned)gdb::ld_rlx(mnw);
+    }
+    __syncthreads();
+    if (T.l->aborted) return -1;
+    ++epoch;
+    if (s_c == 0) {
+      // Nothing moved below the threshold: the next band, or the end.
+      const unsigned gmin = s_gmin;
+      if (gmin >= kInf) {
+        done = true;
+      } else {
+        thresh = (band >= kInf - gmin) ? kInf : gmin + band;
+        drain_far(s & 1, s % 3);
+      }
+    }
+  }
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) out[lo + i] = dist[i];
+  wave_add(&work[0], items);
+  wave_add(&work[1], edges);
+  return s;
+}
+
 // 512-thread groups serve many small tasks at once (throughput-bound): they
 // are held to 80 VGPRs, three groups per CU instead of two, at the price of a
 // few spilled long-lived values.
-template <int WGS, bool SEL>
-__global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(WGS == 512 && !SEL ? 6 : 1))) void k_batch(gdb_kernel_ctx_t* ctx, const DevTask* __restrict__ tasks,
+template <int WGS, bool SEL, bool OWN = false>
+__global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(WGS == 512 && !SEL && !OWN ? 6 : 1))) void k_batch(gdb_kernel_ctx_t* ctx, const DevTask* __restrict__ tasks,
                                                int ntasks, int G, int* team_state, int* next_task,
                                                int* results, int* teams_used,
-                                               unsigned long long* work, gdbsel::Cfg sel, int qcap) {
+                                               unsigned long long* work, gdbsel::Cfg sel, int qcap,
+                                               int region) {
   DISCOVERY_PROTOCOL(ctx);
   __shared__ int gen_cache;
   __shared__ int lcnt[3];
-  // Dynamic LDS: the two local queue buffers (qcap ints each), then the MLP
-  // weights (MLP mode only).
+  // Dynamic LDS: `region` bytes for the local queues (two buffers of qcap
+  // ints) or an owned solve's range, then the MLP weights (MLP mode only).
   extern __shared__ int dyn_lds[];
   int* const lq = dyn_lds;
-  unsigned short* const sel_w = (unsigned short*)(dyn_lds + 2 * qcap);
+  unsigned short* const sel_w = (unsigned short*)((char*)dyn_lds + region);
   const int P = p_get_num_groups(ctx, &local_ctx);
   const int id = p_get_group_id(ctx, &local_ctx);
   if (G > P) G = P;
@@ -1039,7 +1501,14 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(WGS == 512 
       if (Gn < G) valid |= 1u << GDB_ARM_PUSH_NARROW;
       if (pull_ok && Gn < G) valid |= 1u << GDB_ARM_PULL_NARROW;
       const SelRun R{sel, mlp ? sel_w : nullptr, t, Gn, valid};
-      if (k.algo == GDB_TASK_BFS)
+      // Owned solves run in the OWN instantiation only (a batch with an
+      // owned task and no selector), which keeps their registers out of the
+      // other kernels.
+      const bool owned = OWN && k.owned && k.algo != GDB_TASK_COLOR && owned_range(k.n, G) <= 65536 &&
+                         owned_lds_core(owned_range(k.n, G), qcap) <= region;
+      if (owned)
+        rounds = k.algo == GDB_TASK_BFS ? team_owned<true>(T, k, wk, region) : team_owned<false>(T, k, wk, region);
+      else if (k.algo == GDB_TASK_BFS)
         rounds = k.hops > 0 ? team_worklist<true, true, false>(T, k, wk, R)
                             : team_worklist<false, false, SEL>(T, k, wk, R);
       else if (k.algo == GDB_TASK_SSSP) rounds = team_worklist<true, false, SEL>(T, k, wk, R);
@@ -1067,6 +1536,8 @@ struct BatchPlan {
   gdb_ctx* ctx = nullptr;
   int ntasks = 0, G = 1, groups = 0, wgs = 256, qcap = kLocalQDefault;
   size_t dyn_lds = 0;
+  int region = 0;  // bytes of dynamic LDS before the MLP weights
+  bool owned_kernel = false;  // launch the instantiation that runs owned solves
   gdbsel::Cfg cfg{GDB_SEL_NONE, 0, 1, 0u, nullptr, nullptr, 0, nullptr};
   std::vector<int> order;  // order[i] = caller's index of table entry i
   DevTask* d_tasks = nullptr;
@@ -1122,24 +1593,39 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
         (a.n > 0 && !a.row) || (a.m > 0 && !a.col) || (a.algo == GDB_TASK_SSSP && a.m > 0 && !a.w) ||
         (a.algo == GDB_TASK_COLOR && a.n > 0 && !a.node_value) ||
         (a.algo != GDB_TASK_COLOR && a.n > 0 && (a.source < 0 || a.source >= a.n)) || a.delta < 0 ||
-        a.hops < 0 || a.hops > kMaxHops || (a.trow && !a.tcol) ||
+        a.hops < 0 || a.hops > kMaxHops || (a.trow && !a.tcol) || a.owned < 0 || a.owned > 1 ||
         (cfg.mode != GDB_SEL_NONE && a.algo != GDB_TASK_COLOR && a.n > kNinMask))
       return GDB_E_INVALID;
-    const size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : a.algo == GDB_TASK_COLOR ? 5 : 2) * a.n;
-    dt[t] = DevTask{a.algo, a.n, a.m, a.source, a.delta, a.hops, a.row, a.col, a.w, a.node_value, a.out,
+    size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : a.algo == GDB_TASK_COLOR ? 5 : 2) * a.n;
+    // An owned solve keeps its boundary marks (a bit per vertex) there.
+    if (a.owned && a.algo != GDB_TASK_COLOR) ints = std::max(ints, ((size_t)a.n + 31) / 32);
+    dt[t] = DevTask{a.algo, a.n, a.m, a.source, a.delta, a.hops, a.owned, a.row, a.col, a.w, a.node_value, a.out,
                     nullptr, a.trow, a.tcol, a.tw};
     scratch_total += (ints + 63) & ~(size_t)63;
   }
-  // Only local hops use the LDS queue: without them it takes no LDS (more
-  // groups fit a CU).
-  bool any_hops = false;
-  for (int t = 0; t < ntasks; ++t) any_hops |= tasks[t].algo != GDB_TASK_COLOR && tasks[t].hops > 0;
-  if (!any_hops) P->qcap = 0;
-  P->dyn_lds = sizeof(int) * 2 * (size_t)P->qcap +
-               ((cfg.mode == GDB_SEL_MLP && cfg.w) ? sizeof(unsigned short) * gdbsel::kParams : 0);
+  // Only local hops and owned solves use the LDS queue: without them it
+  // takes no LDS (more groups fit a CU).  An owned solve takes what a CU
+  // has left (one group per CU).
+  bool any_hops = false, any_owned = false;
+  for (int t = 0; t < ntasks; ++t) {
+    any_hops |= tasks[t].algo != GDB_TASK_COLOR && tasks[t].hops > 0;
+    any_owned |= tasks[t].algo != GDB_TASK_COLOR && tasks[t].owned;
+  }
+  // With a selector, owned tasks run the shared-queue engine.
+  P->owned_kernel = any_owned && cfg.mode == GDB_SEL_NONE;
+  if (!any_hops && !P->owned_kernel) P->qcap = 0;
+  const size_t sel_bytes = (cfg.mode == GDB_SEL_MLP && cfg.w) ? sizeof(unsigned short) * gdbsel::kParams : 0;
+  P->region = (int)(sizeof(int) * 2 * (size_t)P->qcap);
+  if (P->owned_kernel) P->region = std::max<int>(P->region, (int)kOwnedLdsTotal);
+  P->dyn_lds = (size_t)P->region + sel_bytes;
   int rc = gdb_dispatch_wgs(P->wgs, [&](auto W) -> int {
     constexpr int WG = decltype(W)::value;
-    const void* kern = cfg.mode != GDB_SEL_NONE ? (const void*)k_batch<WG, true> : (const void*)k_batch<WG, false>;
+    const void* kern = cfg.mode != GDB_SEL_NONE ? (const void*)k_batch<WG, true>
+                       : P->owned_kernel        ? (const void*)k_batch<WG, false, true>
+                                                : (const void*)k_batch<WG, false>;
+    // Above 64 KiB of dynamic LDS a kernel has to opt in (a CU has 160 KiB).
+    if (P->dyn_lds > 65536)
+      GDB_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->dyn_lds));
     return launch_groups(ctx, kern, &o, (long long)GDB_MAX_GROUPS * WG, &P->groups, P->dyn_lds);
   });
   if (rc) return rc;
@@ -1194,11 +1680,15 @@ int enqueue_kernel(const BatchPlan& P) {
     if (P.cfg.mode != GDB_SEL_NONE)
       hipLaunchKernelGGL((k_batch<WG, true>), dim3(P.groups), dim3(WG), P.dyn_lds, ctx->stream, ctx->d,
                          P.d_tasks, P.ntasks, P.G, P.team_state, next_task, results, teams_used, P.work, P.cfg,
-                         P.qcap);
+                         P.qcap, P.region);
+    else if (P.owned_kernel)
+      hipLaunchKernelGGL((k_batch<WG, false, true>), dim3(P.groups), dim3(WG), P.dyn_lds, ctx->stream, ctx->d,
+                         P.d_tasks, P.ntasks, P.G, P.team_state, next_task, results, teams_used, P.work, P.cfg,
+                         P.qcap, P.region);
     else
       hipLaunchKernelGGL((k_batch<WG, false>), dim3(P.groups), dim3(WG), P.dyn_lds, ctx->stream, ctx->d,
                          P.d_tasks, P.ntasks, P.G, P.team_state, next_task, results, teams_used, P.work, P.cfg,
-                         P.qcap);
+                         P.qcap, P.region);
     GDB_HIP(hipGetLastError());
     return GDB_SUCCESS;
   });

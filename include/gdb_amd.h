@@ -321,7 +321,15 @@ typedef struct {
   int hops;                 /* BFS / SSSP: local hops a group may run past its share of
                                an iteration before the team barrier, 0..4096
                                (0 = level-synchronous) */
-  const int* trow;          /* in-edge CSR (the transpose; the CSR itself for a symmetric
+  int owned;                /* BFS / SSSP, 0 or 1: owner-partitioned solve.  Group r of the
+                               team owns vertices [r*B, r*B + B), B = ceil(n / team groups)
+                               rounded up to 32, and keeps their distances and row offsets
+                               in LDS; only relaxations between ranges go through device
+                               memory.  Suits graphs whose ids follow locality (grids,
+                               meshes, BFS-ordered graphs).  Needs B <= 65536 and the LDS
+                               of one group per CU; otherwise the task runs as owned = 0.
+                               hops is ignored. */
+  const int* trow;         /* in-edge CSR (the transpose; the CSR itself for a symmetric
                                graph) for pull iterations; NULL = push only */
   const int* tcol;
   const int* tw;            /* in-edge weights (SSSP) */
