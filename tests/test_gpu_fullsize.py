@@ -4,7 +4,8 @@ the benchmarks run.
   grid 2048^2, 5% of edges removed (seed 2), weights U[1,255]: the batched
       engine at the benchmark's tuned shape (wgs 128, 256 groups in two teams
       of 128, near/far band 4096, 256 local hops) and the single-task tuned
-      kernels; BFS levels against the Lonestar worklist restatement,
+      kernels, and the owner-partitioned solve (256 ranges, or two teams of
+      128); BFS levels against the Lonestar worklist restatement,
       distances against Dijkstra, bit for bit
   the same shape on an R-MAT graph whose frontiers overflow the 2048-entry
       LDS queue (the spill path)
@@ -81,6 +82,22 @@ def test_grid2048_batched_at_tuned_shape(ctx, grid2048):
         assert (out_s.to_numpy().astype(np.int64) == want_s).all()
         say(rounds, st.as_dict())
     assert st.participating == TUNED_C2[1]
+
+
+@pytest.mark.parametrize("team,wgs", [(256, 512), (128, 512)])
+def test_grid2048_owned_ranges(ctx, grid2048, team, wgs):
+    # Owner-partitioned solves at full size: 256 ranges of 16384 vertices
+    # (row offsets in LDS) or two teams of 128 ranges of 32768 (without).
+    g, dg, want_b, want_s = grid2048
+    out_b = gdb.DeviceBuffer.empty(g.n, np.uint32)
+    out_s = gdb.DeviceBuffer.empty(g.n, np.uint32)
+    say(f"grid 2048 owned, team {team}, wgs {wgs}, 256 groups")
+    tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0, owned=True),
+             gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=4096, owned=True)]
+    _, st = gdb.batch_solve(ctx, tasks, team, gdb.Opts.make(wgs=wgs, groups=256))
+    assert (out_b.to_numpy() == want_b).all()
+    assert (out_s.to_numpy().astype(np.int64) == want_s).all()
+    say(st.as_dict(), [t.work_items for t in tasks])
 
 
 @pytest.mark.parametrize("algo", ["bfs", "sssp"])
