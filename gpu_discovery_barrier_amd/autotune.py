@@ -242,7 +242,8 @@ def main() -> int:
     merged = dict(old)
     for r in rows:
         merged[(r["solver"], r["n_max"])] = r
-    dev = api.device_query().splitlines()
+    # The first device-query line with a value (the marketing name can be empty).
+    dev = [ln.strip() for ln in api.device_query().splitlines() if ln.split(":", 1)[-1].strip()]
     write_table(list(merged.values()), a.out,
                 header=f"device: {dev[0] if dev else '?'}; tuned in {time.time() - t0:.0f}s"
                        f"{' (quick)' if a.quick else ''}")
