@@ -1000,7 +1000,8 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
 // a fire-and-forget device-scope atomic min on the destination's output
 // word; an owner reads the output words of its boundary vertices (the ones
 // with an in-edge from another range, marked once per task) every few
-// rounds and while it is idle.  (A per-owner counter of such updates, to
+// rounds (a scan costs about 3 us: one device-scope load per boundary
+// vertex) and while it is idle.  (A per-owner counter of such updates, to
 // skip scans that find nothing, measured slower on a 2048^2 grid: every
 // scan there finds something, and the extra atomic per edge costs.)
 //
@@ -1024,8 +1025,17 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
 //
 // LDS (dynamic, `region` bytes): dist[B] u32 | inq[B/32] | far[B/32] |
 // queue u16[2][qcap] | row offsets[B + 1] when they fit | boundary list u16.
-constexpr int kOwnedScanEvery = 2;  // rounds between boundary scans of a busy group
-constexpr int kOwnedBfsBand = 128;  // BFS levels per band
+// Rounds between boundary scans of a busy group, and BFS levels per band,
+// as measured on the 2048^2 grid (scans every 2 / 4 / 8 / 16 rounds, bands
+// of 32 to 512 levels): BFS 8; SSSP 4 when the row offsets are in LDS (256
+// ranges: 19.2 against 20.7 ms at 8), 8 when they are not (128 ranges: 19.5
+// against 20.2 ms at 4).
+constexpr int kOwnedScanEveryBfs = 8, kOwnedScanEverySsspRows = 4, kOwnedScanEverySssp = 8;
+constexpr int kOwnedBfsBand = 256;
+// Queue entries per buffer at most: a larger tile would take the LDS the
+// boundary list needs (a range whose list does not fit scans all of its
+// vertices instead: 23.8 against 19.5 ms at a 4096-entry tile).
+constexpr int kOwnedQueueMax = 2048;
 // Dynamic LDS of a batch with an owned task: a CU's 160 KiB less the kernel's
 // static LDS (a few hundred bytes).
 constexpr long long kOwnedLdsTotal = 163840 - 1024;
@@ -1064,7 +1074,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
   const int B = owned_range(n, G);
   const int lo = min(n, r * B), hi = min(n, lo + B), nb = hi - lo;
   const int W = B >> 5;
-  const int qc = T.qcap;
+  const int qc = min(T.qcap, kOwnedQueueMax);
   lds_char* const lb = (lds_char*)T.lq;
   lds_u32* const dist = (lds_u32*)lb;
   lds_u32* const inq = dist + B;
@@ -1163,6 +1173,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
   const int nscan = scan_all ? nb : nbl;
 
   const unsigned band = UNIT ? (unsigned)kOwnedBfsBand : delta > 0 ? (unsigned)delta : kInf;
+  const int kScanEvery = UNIT ? kOwnedScanEveryBfs : rowl ? kOwnedScanEverySsspRows : kOwnedScanEverySssp;
   unsigned thresh = band;
   long long items = 0, edges = 0;
   // Relaxes local vertex li to cand on lanes with act.  An improvement below
@@ -1347,7 +1358,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
       const int cb = (s + 1) % 3, qa = s & 1, qb = qa ^ 1;
       if (threadIdx.x == 0) s_cnt[(s + 2) % 3] = 0;
       round(qa, ca, qb, cb);
-      if (s % kOwnedScanEvery == kOwnedScanEvery - 1) scan(qb, cb);
+      if (s % kScanEvery == kScanEvery - 1) scan(qb, cb);
       ++s;
       __syncthreads();
     }
@@ -1505,7 +1516,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(WGS == 512 
       // owned task and no selector), which keeps their registers out of the
       // other kernels.
       const bool owned = OWN && k.owned && k.algo != GDB_TASK_COLOR && owned_range(k.n, G) <= 65536 &&
-                         owned_lds_core(owned_range(k.n, G), qcap) <= region;
+                         owned_lds_core(owned_range(k.n, G), min(qcap, kOwnedQueueMax)) <= region;
       if (owned)
         rounds = k.algo == GDB_TASK_BFS ? team_owned<true>(T, k, wk, region) : team_owned<false>(T, k, wk, region);
       else if (k.algo == GDB_TASK_BFS)
