@@ -356,19 +356,48 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
 }
 
 // ---- multi-source BC on a symmetric graph -----------------------------------
-// kMS sources share one sweep.  Per vertex one record of 2 x kMS ints: the
-// kMS sources' levels (-1 unreached), then their sigma bits (forward), then
-// (1 + delta) / sigma bits (backward, once delta is final), as kNR int4 each.
-// One gather of a neighbour's record serves every source, and the sweep
-// reads the column array once for kMS sources.  Step t's queue range holds
-// the vertices that sit on level t for at least one source; walking such a
-// vertex pulls sigma from level t-1 and claims level t+1 for exactly those
-// sources.  The backward pass walks the steps from the deepest down: a
-// vertex on level t of source s takes delta_s = sigma_s * sum of coef_s over
-// neighbours on level t+1 (none on its source's deepest level: delta 0).
+// kMS sources share one sweep, which reads the column array once for all of
+// them.  Step t's queue range holds the vertices that sit on level t for at
+// least one source; walking such a vertex pulls sigma from level t-1 and
+// claims level t+1 for exactly those sources.  The backward pass walks the
+// steps from the deepest down: a vertex on level t of source s takes
+// delta_s = sigma_s * sum of coef_s over neighbours on level t+1 (none on its
+// source's deepest level: delta 0), coef = (1 + delta) / sigma.
+//
+// Per vertex, three arrays:
+//   lev[v]  kMS ints, the sources' levels (-1 unreached): read once per queue
+//           entry, for the vertex's own source mask;
+//   code[v] one word, a nibble per source: bit 3 reached, bits 0-1 the level
+//           mod 4;
+//   sg[v]   kMS floats (32 bytes): sigma in the forward pass, coef once the
+//           vertex's delta is final in the backward pass.
+// On a symmetric graph the levels of two adjacent vertices differ by at most
+// one for every source, so for a vertex on level t a neighbour's nibble tells
+// t-1, t, t+1 and unreached apart.  An edge therefore gathers the 4-byte code
+// word, and the 32-byte sg record only when the neighbour sits on the level
+// the walk pulls from (t-1 forward, t+1 backward) for one of the walking
+// vertex's sources.  A claim of a neighbour for several sources is one
+// fetch-or of their nibbles (every claimer in step t ors the same level-t+1
+// nibble, and the winners are the sources whose reached bit it set); each
+// winner stores the full level in lev.
 constexpr int kMS = 8;
-constexpr int kNR = kMS / 4;             // int4 per half record
-constexpr int kSlotsM = kMS > 4 ? 2 : 4;  // edge slots per lane step (registers)
+constexpr int kNR = kMS / 4;  // int4 per lev / sg record
+constexpr int kSlotsM = 4;    // edge slots per lane step
+constexpr unsigned kNibLo = 0x11111111u;
+
+// Bit 4s of the result is set iff source s's nibble of `code` is reached and
+// holds level `lv` mod 4.
+__device__ __forceinline__ unsigned nib_on_level(unsigned code, unsigned lv) {
+  const unsigned x = (code ^ (lv * kNibLo)) & 0x33333333u;
+  return ~(x | (x >> 1)) & (code >> 3) & kNibLo;
+}
+// Bit s of `mask` moved to bit 4s.
+__device__ __forceinline__ unsigned nib_spread(unsigned mask) {
+  unsigned r = 0;
+#pragma unroll
+  for (int s = 0; s < kMS; ++s) r |= ((mask >> s) & 1u) << (4 * s);
+  return r;
+}
 
 struct FV {  // per-source partial sums
   float v[kMS];
@@ -398,21 +427,18 @@ __device__ __forceinline__ int at(const Half& h, int s) {
   const int k = s & 3;
   return k == 0 ? x.x : k == 1 ? x.y : k == 2 ? x.z : x.w;
 }
-__device__ __forceinline__ void ld_half(const int4* st, size_t rec, int part, Half& h) {
+__device__ __forceinline__ void ld_half(const int4* a, size_t v, Half& h) {
 #pragma unroll
-  for (int r = 0; r < kNR; ++r) h.q[r] = st[rec * (2 * kNR) + part * kNR + r];
+  for (int r = 0; r < kNR; ++r) h.q[r] = a[v * kNR + r];
 }
-__device__ __forceinline__ int* rec_int(int4* st, size_t rec, int part, int s) {
-  return (int*)&st[rec * (2 * kNR) + part * kNR] + s;
-}
+__device__ __forceinline__ int* rec_int(int4* a, size_t v, int s) { return (int*)&a[v * kNR] + s; }
 
-// Gathers in flight bound this kernel: it is held to 96 VGPRs (five waves per
-// SIMD instead of four) for a few spilled values.
+// Gathers in flight bound this kernel.
 template <int WGS>
-__global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k_bc_multi(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
-                                                  const int* __restrict__ col, int4* st, float* bc,
-                                                  const int* sources, int nsrc, int* q, int* stamp,
-                                                  int* reach, int* loff, int* tail, int n, int* out) {
+__global__ __launch_bounds__(WGS) void k_bc_multi(gdb_kernel_ctx_t* ctx, const int* __restrict__ row,
+                                                  const int* __restrict__ col, int4* lev, int4* sg,
+                                                  unsigned* code, float* bc, const int* sources, int nsrc,
+                                                  int* q, int* stamp, int* loff, int* tail, int n, int* out) {
   DISCOVERY_PROTOCOL(ctx);
   const int tid = p_get_global_id(ctx, &local_ctx);
   const int stride = p_get_global_size(ctx, &local_ctx);
@@ -424,11 +450,11 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k
   auto reset = [&](int v) {
 #pragma unroll
     for (int r = 0; r < kNR; ++r) {
-      st[(size_t)v * (2 * kNR) + r] = make_int4(-1, -1, -1, -1);
-      st[(size_t)v * (2 * kNR) + kNR + r] = make_int4(0, 0, 0, 0);
+      lev[(size_t)v * kNR + r] = make_int4(-1, -1, -1, -1);
+      sg[(size_t)v * kNR + r] = make_int4(0, 0, 0, 0);
     }
+    code[v] = 0u;
     stamp[v] = -1;
-    reach[v] = 0;
   };
   for (int i = tid; i < n; i += stride) {
     bc[i] = 0.f;
@@ -447,9 +473,9 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k
       int cnt = 0;
       for (int j = 0; j < ns; ++j) {
         const int s = sources[s0 + j];
-        *rec_int(st, s, 0, j) = 0;
-        *rec_int(st, s, 1, j) = __float_as_int(1.f);
-        reach[s] |= 1 << j;
+        *rec_int(lev, s, j) = 0;
+        *rec_int(sg, s, j) = __float_as_int(1.f);
+        code[s] |= 8u << (4 * j);  // reached, level 0
         if (stamp[s] != 0) {
           stamp[s] = 0;
           q[cnt++] = s;
@@ -465,6 +491,8 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k
     // Forward.
     int lb = 0, le = gdb::ld_rlx(tail), t = 0;
     for (;;) {
+      const unsigned par = (unsigned)(t - 1) & 3u;                    // parents' level mod 4
+      const unsigned nxt_nib = 8u | ((unsigned)(t + 1) & 3u);         // a claimed nibble
       GDB_FOR_UNIFORM(i, valid, tid, stride, le - lb) {
         int v = -1, b = 0, e = 0;
         unsigned mask = 0;
@@ -474,7 +502,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k
         }
         if (valid) {
           Half lv;
-          ld_half(st, v, 0, lv);
+          ld_half(lev, v, lv);
 #pragma unroll
           for (int s = 0; s < kMS; ++s) mask |= (at(lv, s) == t ? 1u : 0u) << s;
           mask &= all;
@@ -483,7 +511,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k
         }
         // On a long row every lane works for the row's owner: `om` is the
         // owner's source mask.
-        const FV sg = gdb::wave_walk_np<kSlotsM>(
+        const FV sgm = gdb::wave_walk_np<kSlotsM>(
             valid && mask != 0, b, e, (int)mask, fv_zero(),
             [&](const bool (&ok)[kSlotsM], const int (&kk)[kSlotsM], int om) -> FV {
               int w[kSlotsM];
@@ -497,51 +525,52 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k
                 any |= ok[j];
               }
               if (any) {  // lanes without a slot this step stay out
-                Half lw[kSlotsM], sw[kSlotsM];
-                bool in[kSlotsM];
+                const unsigned omn = nib_spread((unsigned)om);
+                unsigned c[kSlotsM], pm[kSlotsM], cl[kSlotsM];
 #pragma unroll
                 for (int j = 0; j < kSlotsM; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
 #pragma unroll
                 for (int j = 0; j < kSlotsM; ++j) {
-                  in[j] = ok[j] && (unsigned)w[j] < (unsigned)n;
-                  const size_t r = (size_t)(in[j] ? w[j] : spare);
-                  ld_half(st, r, 0, lw[j]);
-                  ld_half(st, r, 1, sw[j]);
+                  const bool in = ok[j] && (unsigned)w[j] < (unsigned)n;
+                  c[j] = gdb::ldg(code + (in ? w[j] : spare));
+                  if (!in) w[j] = spare;
+                  const unsigned sel = in ? omn : 0u;
+                  pm[j] = nib_on_level(c[j], par) & sel;  // neighbour on level t-1
+                  cl[j] = (~c[j] >> 3) & kNibLo & sel;    // neighbour unreached
                 }
-                unsigned claim[kSlotsM];
+                // sigma of the neighbours on level t-1, where some source has one.
+                Half sw[kSlotsM];
+#pragma unroll
+                for (int j = 0; j < kSlotsM; ++j)
+                  if (pm[j]) ld_half(sg, w[j], sw[j]);
+#pragma unroll
+                for (int j = 0; j < kSlotsM; ++j)
+                  if (pm[j])
+#pragma unroll
+                    for (int s = 0; s < kMS; ++s)
+                      if ((pm[j] >> (4 * s)) & 1u) acc.v[s] += __int_as_float(at(sw[j], s));
                 bool any_claim = false;
 #pragma unroll
-                for (int j = 0; j < kSlotsM; ++j) {
-                  claim[j] = 0;
-#pragma unroll
-                  for (int s = 0; s < kMS; ++s) {
-                    const int ls = at(lw[j], s);
-                    const bool mine = in[j] && ((om >> s) & 1);
-                    if (mine && ls == t - 1) acc.v[s] += __int_as_float(at(sw[j], s));
-                    claim[j] |= (mine && ls == -1 ? 1u : 0u) << s;
-                  }
-                  any_claim |= claim[j] != 0;
-                }
-                // Only lanes with an unreached target send claims.  One
-                // fetch-or on the target's reached mask claims all of the
-                // slot's sources at once (winners are the bits it sets), and
-                // each winner writes its level with a plain store: a reader
-                // that still sees -1 there loses at the mask.
+                for (int j = 0; j < kSlotsM; ++j) any_claim |= cl[j] != 0u;
+                // Only lanes with an unreached target send claims: one fetch-or
+                // of the claimed nibbles (winners are the sources whose reached
+                // bit it set), then each winner stores the full level.
                 if (any_claim) {
-                  unsigned oldm[kSlotsM];
+                  unsigned oldc[kSlotsM];
 #pragma unroll
                   for (int j = 0; j < kSlotsM; ++j)
-                    oldm[j] = (unsigned)gdb::atomic_or_g(&reach[claim[j] ? w[j] : spare], (int)claim[j]);
+                    oldc[j] = (unsigned)gdb::atomic_or_g((int*)&code[cl[j] ? w[j] : spare],
+                                                         (int)(cl[j] * nxt_nib));
                   bool won[kSlotsM];
                   bool any_won = false;
 #pragma unroll
                   for (int j = 0; j < kSlotsM; ++j) {
-                    const unsigned got = claim[j] & ~oldm[j];
+                    const unsigned got = cl[j] & ~(oldc[j] >> 3);
                     won[j] = got != 0u;
                     any_won |= won[j];
 #pragma unroll
                     for (int s = 0; s < kMS; ++s)
-                      if ((got >> s) & 1u) *rec_int(st, w[j], 0, s) = t + 1;
+                      if ((got >> (4 * s)) & 1u) *rec_int(lev, w[j], s) = t + 1;
                   }
                   // A claimed vertex joins step t+1's queue once.
                   if (any_won) {
@@ -561,7 +590,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k
         if (valid && t > 0) {
 #pragma unroll
           for (int s = 0; s < kMS; ++s)
-            if ((mask >> s) & 1u) *rec_int(st, v, 1, s) = __float_as_int(sg.v[s]);
+            if ((mask >> s) & 1u) *rec_int(sg, v, s) = __float_as_int(sgm.v[s]);
         }
       }
       if (!discovery_barrier(ctx, &local_ctx)) return;
@@ -595,38 +624,41 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k
         }
         if (valid) {
           Half lv;
-          ld_half(st, v, 0, lv);
-          ld_half(st, v, 1, sv);
+          ld_half(lev, v, lv);
+          ld_half(sg, v, sv);
 #pragma unroll
           for (int s = 0; s < kMS; ++s) mask |= (at(lv, s) == tt ? 1u : 0u) << s;
           mask &= all;
           b = row[v];
           e = row[v + 1];
         }
+        const unsigned chi = (unsigned)(tt + 1) & 3u;  // children's level mod 4
         const FV acc = gdb::wave_walk_np<kSlotsM>(
             valid && mask != 0, b, e, (int)mask, fv_zero(),
             [&](const bool (&ok)[kSlotsM], const int (&kk)[kSlotsM], int om) -> FV {
               FV a = fv_zero();
               if (!gdb::any_slot(ok)) return a;  // no slot: no loads
+              const unsigned omn = nib_spread((unsigned)om);
               int w[kSlotsM];
-              Half lw[kSlotsM], sw[kSlotsM];
+              unsigned cm[kSlotsM];
+              Half sw[kSlotsM];
 #pragma unroll
               for (int j = 0; j < kSlotsM; ++j) w[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
 #pragma unroll
               for (int j = 0; j < kSlotsM; ++j) {
                 const bool in = ok[j] && (unsigned)w[j] < (unsigned)n;
-                const size_t r = (size_t)(in ? w[j] : spare);
-                ld_half(st, r, 0, lw[j]);
-                ld_half(st, r, 1, sw[j]);
-                if (!in)
-#pragma unroll
-                  for (int rr = 0; rr < kNR; ++rr) lw[j].q[rr] = make_int4(-1, -1, -1, -1);
+                cm[j] = nib_on_level(gdb::ldg(code + (in ? w[j] : spare)), chi) & (in ? omn : 0u);
               }
+              // coef of the neighbours on level tt+1, where some source has one.
 #pragma unroll
               for (int j = 0; j < kSlotsM; ++j)
+                if (cm[j]) ld_half(sg, w[j], sw[j]);
 #pragma unroll
-                for (int s = 0; s < kMS; ++s)
-                  if (((om >> s) & 1) && at(lw[j], s) == tt + 1) a.v[s] += __int_as_float(at(sw[j], s));
+              for (int j = 0; j < kSlotsM; ++j)
+                if (cm[j])
+#pragma unroll
+                  for (int s = 0; s < kMS; ++s)
+                    if ((cm[j] >> (4 * s)) & 1u) a.v[s] += __int_as_float(at(sw[j], s));
               return a;
             },
             [](FV x, FV c) { return fv_add(x, c); }, kSerialLimit);
@@ -638,7 +670,7 @@ __global__ __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(5))) void k
               const float sw = __int_as_float(at(sv, s));
               const float dw = sw * acc.v[s];
               if (v != sources[s0 + s]) add += dw;
-              *rec_int(st, v, 1, s) = __float_as_int((1.f + dw) / sw);
+              *rec_int(sg, v, s) = __float_as_int((1.f + dw) / sw);
             }
           bc[v] += add;
         }
@@ -792,9 +824,10 @@ extern "C" int gdb_bc(gdb_ctx* ctx, int n, int m, const int* row, const int* col
       if (r2) return r2;
       r2 = timer_start(ctx);
       if (r2) return r2;
-      if (multi)  // d holds the per-vertex queue stamps and q the reached masks here
+      if (multi)  // mst holds lev then sg; d the per-vertex queue stamps; q the level codes
         hipLaunchKernelGGL(k_bc_multi<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col, mst,
-                           bc, sources, nsrc, mq, d, q, mloff, flags + 4, n, out);
+                           mst + (size_t)kNR * n, (unsigned*)q, bc, sources, nsrc, mq, d, mloff, flags + 4, n,
+                           out);
       else if (tuned)
         hipLaunchKernelGGL(k_bc_tuned<WG>, dim3(groups), dim3(WG), 0, ctx->stream, ctx->d, row, col,
                            row_t, col_t, ds, bc, sources, nsrc, q, flags + 4, n, out);
