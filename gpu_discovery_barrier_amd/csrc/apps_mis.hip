@@ -125,27 +125,34 @@ __global__ __launch_bounds__(WGS) void k_mis_tuned(gdb_kernel_ctx_t* ctx, const 
         b = row[v];
         e = row[v + 1];
       }
-      // kSlots edges per lane step: the column loads, then the 8-byte
-      // (status, value) gathers, then the compares.
-      auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> float {
-        if (!gdb::any_slot(ok)) return kBigNum;  // no slot: no loads
+      // v joins iff its value is below kBigNum and below every undecided
+      // neighbour's (the reference's minimum, lines 240-266, compared with
+      // <): the walk stops at the first undecided neighbour whose value is
+      // not above v's.  kSlots edges per lane step: the column loads, then
+      // the 8-byte (status, value) gathers, then the compares.
+      const float vv = valid ? __int_as_float(cv[v].y) : kBigNum;
+      auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots], int own) -> bool {
+        if (!gdb::any_slot(ok)) return false;  // no slot: no loads
+        const float ov = __int_as_float(own);
         int u[kSlots];
         long long x[kSlots];
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) u[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));  // edge 0 exists here
 #pragma unroll
-        for (int j = 0; j < kSlots; ++j) x[j] = gdb::ldg((const long long*)(cv + (ok[j] ? u[j] : spare)));
-        float m = kBigNum;
-#pragma unroll
         for (int j = 0; j < kSlots; ++j) {
-          const float vj = __int_as_float((int)(x[j] >> 32));
-          if (ok[j] && (int)(x[j] & 0xffffffffll) == -1 && vj < m) m = vj;
+          const bool in = ok[j] && (unsigned)u[j] < (unsigned)n;
+          x[j] = gdb::ldg((const long long*)(cv + (in ? u[j] : spare)));
+          if (!in) x[j] = 0;  // status 0: not undecided
         }
-        return m;
+        bool hit = false;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j)
+          hit |= (int)(x[j] & 0xffffffffll) == -1 && __int_as_float((int)(x[j] >> 32)) <= ov;
+        return hit;
       };
-      auto red = [](float a, float b2) { return a < b2 ? a : b2; };
-      const float mn = gdb::wave_walk_n<kSlots>(valid != 0, b, e, kBigNum, step, red, kSerialLimit);
-      if (valid && __int_as_float(cv[v].y) < mn) s[v] = 2;
+      const bool blocked =
+          gdb::wave_walk_any<kSlots>(valid != 0, b, e, __float_as_int(vv), step, kSerialLimit);
+      if (valid && vv < kBigNum && !blocked) s[v] = 2;
     }
     if (tid == 0) gdb::st_rlx(&tails[(rounds + 1) % 3], 0);
     if (!discovery_barrier(ctx, &local_ctx)) return;
@@ -166,20 +173,24 @@ __global__ __launch_bounds__(WGS) void k_mis_tuned(gdb_kernel_ctx_t* ctx, const 
           e = row[v + 1];
         }
       }
-      auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots]) -> int {
-        if (!gdb::any_slot(ok)) return 0;  // no slot: no loads
+      // A joining neighbour retires v: the walk stops at the first one.
+      auto step = [&](const bool (&ok)[kSlots], const int (&kk)[kSlots], int) -> bool {
+        if (!gdb::any_slot(ok)) return false;  // no slot: no loads
         int u[kSlots], sj[kSlots];
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) u[j] = gdb::ldg(col + (ok[j] ? kk[j] : 0));
 #pragma unroll
-        for (int j = 0; j < kSlots; ++j) sj[j] = gdb::ldg(s + (ok[j] ? u[j] : spare));
-        int h = 0;
+        for (int j = 0; j < kSlots; ++j) {
+          const bool in = ok[j] && (unsigned)u[j] < (unsigned)n;
+          sj[j] = gdb::ldg(s + (in ? u[j] : spare));
+          if (!in) sj[j] = 0;
+        }
+        bool h = false;
 #pragma unroll
-        for (int j = 0; j < kSlots; ++j) h |= (ok[j] && sj[j] == 2) ? 1 : 0;
+        for (int j = 0; j < kSlots; ++j) h |= sj[j] == 2;
         return h;
       };
-      auto red = [](int a, int b2) { return a | b2; };
-      const int hit = gdb::wave_walk_n<kSlots>(valid && !joined, b, e, 0, step, red, kSerialLimit);
+      const bool hit = gdb::wave_walk_any<kSlots>(valid && !joined, b, e, 0, step, kSerialLimit);
       const bool retire = valid && (joined || hit);
       if (retire) cv[v].x = -2;
       const bool keep = valid && !retire;

@@ -270,6 +270,56 @@ __device__ __forceinline__ T wave_walk_np(bool active, int b, int e, int payload
   return acc;
 }
 
+// Chunked cooperative "some edge of [b, e) has pred" with early exit:
+// step(ok, kk, owner) tests N edge slots of the calling lane at once (the
+// same slot layout and payload as wave_walk_np) and returns whether one of
+// them hit.  A lane walking its own short range stops at its first step with
+// a hit; a long range walked by the whole wave stops at the first step in
+// which some lane hit.  All 64 lanes must call it.
+template <int N, typename Step>
+__device__ __forceinline__ bool wave_walk_any(bool active, int b, int e, int payload, Step step,
+                                              int serial_limit) {
+  const bool heavy = active && (e - b) > serial_limit;
+  const int len = (active && !heavy) ? (e - b) : 0;
+  bool found = false;
+  for (int c = 0; __ballot(c < len && !found) != 0ull; c += N) {
+    bool ok[N];
+    int kk[N];
+    const bool live = c < len && !found;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      ok[j] = live && c + j < len;
+      kk[j] = b + c + j;
+    }
+    found |= step(ok, kk, payload);
+  }
+  unsigned long long mask = __ballot(heavy);
+  const int lane = lane_id();
+  while (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    const int lb = bcast(b, leader);
+    const int le = bcast(e, leader);
+    const int lp = bcast(payload, leader);
+    bool hit = false;
+    for (int base = lb; base < le; base += 64 * N) {
+      bool ok[N];
+      int kk[N];
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        kk[j] = base + lane + 64 * j;
+        ok[j] = kk[j] < le;
+      }
+      if (__ballot(step(ok, kk, lp)) != 0ull) {
+        hit = true;
+        break;
+      }
+    }
+    if (lane == leader) found = hit;
+    mask &= mask - 1;
+  }
+  return found;
+}
+
 template <int N, typename T, typename Step, typename R>
 __device__ __forceinline__ T wave_walk_n(bool active, int b, int e, T identity, Step step, R red,
                                          int serial_limit) {
