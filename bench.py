@@ -32,22 +32,27 @@ HBM_PEAK_GBS = 8000.0
 GRID_SIDE = 2048
 
 
-def tuned_c2():
+def tuned_c2(engine: str = ""):
     """The batched engine's shape for the grid from the library's tuning table
     (gpu_discovery_barrier_amd/tune_table.txt, written by the autotuner):
-    ((wgs, groups, near/far band, local hops, selector mode), team groups,
-    LDS queue tile)."""
+    ((wgs, groups, band, local hops, selector mode), team groups, LDS queue
+    tile, owned).  The autotuner writes a "batch_owned" row only when the
+    owner-partitioned engine beat the shared one on this grid; that row then
+    names the headline engine.  engine = "owned" / "shared" forces one."""
     from gpu_discovery_barrier_amd import autotune
 
-    r = autotune.lookup("batch", GRID_SIDE * GRID_SIDE)
+    n = GRID_SIDE * GRID_SIDE
+    r = autotune.lookup("batch_owned", n) if engine != "shared" else None
+    if r is not None or engine == "owned":
+        if r is None:
+            return (1024, 256, 6144, 0, 0), 128, 2048, True
+        return ((r["wgs"], r["groups"], r["delta"], 0, 0), r["team_groups"] or max(1, r["groups"] // 2),
+                r["lds_queue"] or 2048, True)
+    r = autotune.lookup("batch", n)
     if r is None:
-        return (128, 256, 4096, 256, 0), 128, 2048
+        return (128, 256, 4096, 256, 0), 128, 2048, False
     return ((r["wgs"], r["groups"], r["delta"], r["hops"], 0), r["team_groups"] or max(1, r["groups"] // 2),
-            r["lds_queue"] or 2048)
-
-
-# wgs, groups, near/far band, local hops, selector mode (0 none, 4 MLP)
-TUNED_C2, TUNED_C2_TEAM, TUNED_C2_QCAP = tuned_c2()
+            r["lds_queue"] or 2048, False)
 
 
 def log(*a):
@@ -525,6 +530,8 @@ def main() -> int:
                     help="c2: rerun the autotuner (launch shape, band, hops, selector) instead of the tuned shape")
     ap.add_argument("--shape", default="",
                     help="c2: wgs,groups,delta,hops,selector (skips the autotuner; selector 0 none, 4 MLP)")
+    ap.add_argument("--engine", default="", choices=["", "owned", "shared"],
+                    help="c2: owner-partitioned ranges or the shared engine (default: the tuning table's pick)")
     ap.add_argument("--engine-only", action="store_true",
                     help="c2 / c3: time the engine's kernels only (profiler runs)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "copy", "probe", "sched"],
@@ -573,13 +580,16 @@ def main() -> int:
             return None
         return gdb.make_selector(mode, narrow_groups=narrow, weights=weights)
 
-    team, qcap = TUNED_C2_TEAM, TUNED_C2_QCAP
+    tuned_shape, team, qcap, owned = tuned_c2(args.engine)
+    if args.tune or args.shape:
+        owned = args.engine == "owned"
 
     def solve_batched(wgs, groups, delta, hops=0, mode=gdb.SEL_NONE, team_groups=None, lds_queue=None):
-        # One launch, two teams: BFS and near/far SSSP side by side.  The
-        # grid is symmetric, so it is its own in-edge CSR for pull arms.
-        tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0, hops=hops, transpose=dg),
-                 gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta, hops=hops, transpose=dg)]
+        # One launch, two teams: BFS and SSSP side by side.  The grid is
+        # symmetric, so it is its own in-edge CSR for pull arms.  Owned:
+        # every group of a team holds a contiguous vertex range in LDS.
+        tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0, hops=hops, transpose=dg, owned=owned),
+                 gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta, hops=hops, transpose=dg, owned=owned)]
         rounds, st = gdb.batch_solve(ctx, tasks, team_groups or team,
                                      gdb.Opts.make(wgs=wgs, groups=groups, lds_queue=lds_queue or qcap),
                                      selector=selector(mode))
@@ -594,15 +604,17 @@ def main() -> int:
         wgs, groups, delta, hops, mode = (int(x) for x in args.shape.split(","))
         team = max(1, groups // 2)
     elif not args.tune:
-        wgs, groups, delta, hops, mode = TUNED_C2
+        wgs, groups, delta, hops, mode = tuned_shape
     else:
         from gpu_discovery_barrier_amd import autotune
 
-        r, trials = autotune.tune_batch_grid(ctx)
+        rows, trials = autotune.tune_batch_grid(ctx)
+        r = rows[-1]  # the owned row when it won
+        owned = r["solver"] == "batch_owned"
         wgs, groups, team, delta, hops, qcap = r["wgs"], r["groups"], r["team_groups"], r["delta"], r["hops"], \
             r["lds_queue"]
         mode = gdb.SEL_NONE
-        if weights is not None:
+        if weights is not None and not owned:  # the selector's arms belong to the shared engine
             ms_mlp = min(solve_batched(wgs, groups, delta, 0, gdb.SEL_MLP)[1].kernel_ms for _ in range(2))
             if ms_mlp < min(t[0] for t in trials):
                 hops, mode = 0, gdb.SEL_MLP
@@ -626,7 +638,7 @@ def main() -> int:
     if args.engine_only:
         # Only the batched launches (for profiler runs): no comparisons.
         print(json.dumps({"kernel_ms": round(t_step_ms, 3), "gteps": round(gteps, 4),
-                          "shape": [wgs, groups, delta, hops, mode],
+                          "shape": [wgs, groups, delta, hops, mode, int(owned)],
                           "work_edges": bst.work_edges, "work_items": bst.work_items}), flush=True)
         ctx.close()
         return 0
@@ -643,14 +655,17 @@ def main() -> int:
     nerr = gdb.verify_distances(ctx, dg, gdb.DeviceBuffer.from_numpy(ds), weighted=True)
     nerr += gdb.verify_distances(ctx, dg, gdb.DeviceBuffer.from_numpy(db), weighted=False)
 
-    # Barrier vs relaunch: every barrier kind at the team shape (one team =
-    # groups / 2) and at 8, 32, 256 and 1024 groups; relaunch (i) back to back,
-    # (ii) with the reference host loop's blocking readback.
-    P = team
-    bars = barrier_table(ctx, wgs, sorted({8, 32, P, 256, 1024}))
+    # Barrier vs relaunch: every barrier kind at the team's group count (one
+    # team = groups / 2) and at 8, 32, 256 and 1024 groups, all of 128
+    # threads (the shared engine's team shape; the owned engine's rounds use
+    # workgroup barriers, and its team barrier only at set-up and band ends);
+    # relaunch (i) back to back, (ii) with the reference host loop's blocking
+    # readback.
+    P, bwgs = team, 128
+    bars = barrier_table(ctx, bwgs, sorted({8, 32, P, 256, 1024}))
     us_bar = bars["counter" if P < 64 else "tree"][str(P)]["us_per_barrier"]
     us_relaunch = bars["relaunch"][str(P)]
-    us_relaunch_sync = ctx.relaunch_bench(wgs, P, 500, 1) * 1e3 / 500
+    us_relaunch_sync = ctx.relaunch_bench(bwgs, P, 500, 1) * 1e3 / 500
 
     # Roofline of the dominant kernel (the batched kernel).  Algorithmic bytes
     # as the kernels count their work: BFS 8 B per processed edge (dst, dist)
@@ -684,7 +699,10 @@ def main() -> int:
             "workload": f"Lonestar worklist BFS + SSSP, {GRID_SIDE}x{GRID_SIDE} 4-neighbour grid, "
                         "5% edges removed (seed 2), weights U[1,255], source 0",
             "n": g.n, "m": g.m, "m_reached": mr,
-            "engine": "batched (BFS and near/far SSSP teams in one persistent launch)",
+            "engine": ("batched, owner-partitioned ranges (BFS and delta-band SSSP teams in one persistent "
+                       "launch; each group holds a contiguous vertex range's distances in LDS)" if owned else
+                       "batched (BFS and near/far SSSP teams in one persistent launch)"),
+            "owned": owned,
             "wgs": wgs, "groups": groups, "team_groups": team, "delta": delta,
             "local_hops": hops, "lds_queue": qcap, "shape_source": "tuning table" if not (args.shape or args.tune)
             else ("--shape" if args.shape else "--tune"),
@@ -704,14 +722,14 @@ def main() -> int:
         "barrier": {"kind_at_team_shape": "counter" if P < 64 else "tree",
                     "us_per_barrier": round(us_bar, 3), "us_per_relaunch": round(us_relaunch, 3),
                     "us_per_relaunch_with_readback": round(us_relaunch_sync, 3),
-                    "ratio": round(us_bar / us_relaunch, 3), "P": P, "wgs": wgs, "table": bars},
+                    "ratio": round(us_bar / us_relaunch, 3), "P": P, "wgs": bwgs, "table": bars},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": traffic_from_profiles((wgs, groups, delta, hops, mode)),
+            "traffic": traffic_from_profiles((wgs, groups, delta, hops, mode, int(owned))),
             "kernel": "k_batch<WGS> (BFS + SSSP teams)",
             "algorithmic_bytes": int(alg_bytes),
             "minimal_bytes": int(min_bytes),

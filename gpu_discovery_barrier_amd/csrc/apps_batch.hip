@@ -1565,9 +1565,16 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
   gdb_opts o{};
   if (opts) o = *opts;
   {
+    // A batch with an owner-partitioned task takes the "batch_owned" row
+    // when the table has one (one group per CU), else the "batch" row.
     long long nmax = 0;
-    for (int t = 0; t < ntasks; ++t) nmax = std::max<long long>(nmax, tasks[t].n);
-    apply_tuning("batch", nmax, &o, &team_groups);
+    bool any_owned = false;
+    for (int t = 0; t < ntasks; ++t) {
+      nmax = std::max<long long>(nmax, tasks[t].n);
+      any_owned |= tasks[t].owned != 0;
+    }
+    if (!any_owned || !apply_tuning("batch_owned", nmax, &o, &team_groups))
+      apply_tuning("batch", nmax, &o, &team_groups);
   }
   P->ctx = ctx;
   P->ntasks = ntasks;

@@ -261,18 +261,19 @@ bool lookup_tune_locked(const char* solver, long long n, gdb_tune_entry* out) {
 }
 }  // namespace
 
-void apply_tuning(const char* solver, long long n, gdb_opts* o, int* team_groups) {
-  if (o->wgs != 0) return;
+bool apply_tuning(const char* solver, long long n, gdb_opts* o, int* team_groups) {
+  if (o->wgs != 0) return false;
   gdb_tune_entry e{};
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     if (!g_tune_loaded) load_tune_locked(nullptr);
-    if (!lookup_tune_locked(solver, n, &e)) return;
+    if (!lookup_tune_locked(solver, n, &e)) return false;
   }
   o->wgs = e.wgs;
   if (o->groups == 0) o->groups = e.groups;
   if (o->lds_queue == 0) o->lds_queue = e.lds_queue;
   if (team_groups && *team_groups == 0) *team_groups = e.team_groups;
+  return true;
 }
 
 static int init_ctx(gdb_ctx* ctx, int skip) {

@@ -88,8 +88,9 @@ int read_participating(gdb_ctx* ctx, int* out);
 // it open (o->wgs == 0): wgs, groups (if 0) and lds_queue (if 0) from the row
 // for `solver` with the smallest n_max >= n.  `team_groups` (may be NULL)
 // is filled when it points to 0.  Solvers other than the batched engine
-// call it for the tuned persistent variant only.
-void apply_tuning(const char* solver, long long n, gdb_opts* o, int* team_groups = nullptr);
+// call it for the tuned persistent variant only.  Returns whether a row was
+// applied (false when the shape was given or the table has no such row).
+bool apply_tuning(const char* solver, long long n, gdb_opts* o, int* team_groups = nullptr);
 
 // Resolves, once per device, the device address of the static scratch array
 // of the allocation-free worklist entry point and stores it in the context

@@ -2,11 +2,13 @@
 the benchmarks run.
 
   grid 2048^2, 5% of edges removed (seed 2), weights U[1,255]: the batched
-      engine at the benchmark's tuned shape (wgs 128, 256 groups in two teams
-      of 128, near/far band 4096, 256 local hops) and the single-task tuned
-      kernels, and the owner-partitioned solve (256 ranges, or two teams of
-      128); BFS levels against the Lonestar worklist restatement,
-      distances against Dijkstra, bit for bit
+      engine at the benchmark's two tuned shapes (shared: wgs 128, 256 groups
+      in two teams of 128, near/far band 4096, 256 local hops; owned: two
+      teams of 128 vertex ranges, one group of 1024 threads per CU, band
+      6144), the single-task tuned kernels, and the owner-partitioned solve
+      at other shapes (256 ranges, or two teams of 128); BFS levels against
+      the Lonestar worklist restatement, distances against Dijkstra, bit for
+      bit
   the same shape on an R-MAT graph whose frontiers overflow the 2048-entry
       LDS queue (the spill path)
   R-MAT scale 22 (seed 3): MIS against greedy sequential MIS (bit for bit);
@@ -31,7 +33,13 @@ import oracle
 pytestmark = pytest.mark.gpu
 ROOT = Path(__file__).resolve().parents[1]
 sys.path.insert(0, str(ROOT))
-from bench import TUNED_C2  # noqa: E402
+from bench import tuned_c2  # noqa: E402
+
+# The benchmark's two engine shapes from the tuning table: the shared engine
+# and the owner-partitioned one (the headline runs the faster, which the
+# autotuner recorded).
+TUNED_C2, _, _, _ = tuned_c2("shared")
+OWNED_C2, OWNED_TEAM, OWNED_QCAP, _ = tuned_c2("owned")
 
 BC_RTOL, BC_ATOL = 1e-4, 1e-6
 
@@ -82,6 +90,22 @@ def test_grid2048_batched_at_tuned_shape(ctx, grid2048):
         assert (out_s.to_numpy().astype(np.int64) == want_s).all()
         say(rounds, st.as_dict())
     assert st.participating == TUNED_C2[1]
+
+
+def test_grid2048_owned_at_bench_shape(ctx, grid2048):
+    g, dg, want_b, want_s = grid2048
+    wgs, groups, delta, _, _ = OWNED_C2
+    out_b = gdb.DeviceBuffer.empty(g.n, np.uint32)
+    out_s = gdb.DeviceBuffer.empty(g.n, np.uint32)
+    for rep in range(2):
+        say(f"grid 2048 owned at the bench shape {OWNED_C2}, team {OWNED_TEAM}, repeat {rep}")
+        tasks = [gdb.make_task(gdb.TASK_BFS, dg, out_b, 0, owned=True),
+                 gdb.make_task(gdb.TASK_SSSP, dg, out_s, 0, delta=delta, owned=True)]
+        _, st = gdb.batch_solve(ctx, tasks, OWNED_TEAM,
+                                gdb.Opts.make(wgs=wgs, groups=groups, lds_queue=OWNED_QCAP))
+        assert (out_b.to_numpy() == want_b).all()
+        assert (out_s.to_numpy().astype(np.int64) == want_s).all()
+        say(st.as_dict(), [t.work_items for t in tasks])
 
 
 @pytest.mark.parametrize("team,wgs", [(256, 512), (128, 512)])
