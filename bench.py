@@ -2,14 +2,17 @@
 4-neighbour grid (5% of edges removed, seed 2, weights U[1,255], source 0),
 on one MI355X.
 
-A step is one BFS solve plus one SSSP solve with the graph resident in HBM.
-value = GTEPS of the tuned persistent engine = 2 * m_r / (BFS + SSSP kernel
+A step is one BFS solve plus one SSSP solve with the graph resident in HBM,
+both in one persistent launch of the batched engine (two teams side by side;
+the engine and shape the tuning table names: owner-partitioned vertex ranges
+in LDS when the autotuner found them faster).  value = 2 * m_r / (kernel
 time), with m_r = sum of out-degrees of reached vertices.  The plain
-persistent engine (the reference kernels' structure, default launch) runs on
-the same input; outputs of both engines are compared bit for bit and checked
-with dverifysolution.  Also reported: us per barrier vs us per relaunch at the
-tuned launch shape, the roofline of the dominant kernel, and a CPU baseline
-(the oracle's single-thread restatement on the same graph).
+persistent engine (the reference kernels' structure, default launch) and the
+tuned single-task kernels run on the same input; all outputs are compared bit
+for bit and checked with dverifysolution.  Also reported: us per barrier vs
+us per relaunch, the roofline of the dominant kernel with its measured HBM
+traffic, and a CPU baseline (the oracle's threaded restatement, and its
+single-thread time beside it).
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--config c2]
 (--config c1 / c3 / c4 measure the other configurations the same way.)
@@ -361,7 +364,8 @@ def bench_c4(args) -> int:
         for dg, nv, o in zip(dgs, nvs, outs):
             ts.append(gdb.make_task(gdb.TASK_BFS, dg, o[0], 0))
             ts.append(gdb.make_task(gdb.TASK_SSSP, dg, o[1], 0, delta=delta))
-            ts.append(gdb.make_task(gdb.TASK_COLOR, dg, o[2], 0, node_value=nv))
+            # Symmetric graphs: the CSR is its own in-edge CSR.
+            ts.append(gdb.make_task(gdb.TASK_COLOR, dg, o[2], 0, node_value=nv, transpose=dg))
         return ts
 
     def run(wgs, groups, team, delta):

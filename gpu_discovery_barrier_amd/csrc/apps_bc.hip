@@ -382,7 +382,11 @@ __global__ __launch_bounds__(WGS) void k_bc_tuned(gdb_kernel_ctx_t* ctx, const i
 // winner stores the full level in lev.
 constexpr int kMS = 8;
 constexpr int kNR = kMS / 4;  // int4 per lev / sg record
-constexpr int kSlotsM = 4;    // edge slots per lane step
+// Edge slots per lane step: the gathers of a step are in flight together.
+// Measured on R-MAT 22, 64 sources (same box): 4 slots 450-465 ms (4 waves
+// per SIMD), 6: 395, 8: 355-363, 10: 340-346, 12: 331-340 (238 VGPRs, 2
+// waves per SIMD, no spills): gathers in flight per lane beat occupancy.
+constexpr int kSlotsM = 12;
 constexpr unsigned kNibLo = 0x11111111u;
 
 // Bit 4s of the result is set iff source s's nibble of `code` is reached and
