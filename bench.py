@@ -364,8 +364,7 @@ def bench_c4(args) -> int:
         for dg, nv, o in zip(dgs, nvs, outs):
             ts.append(gdb.make_task(gdb.TASK_BFS, dg, o[0], 0))
             ts.append(gdb.make_task(gdb.TASK_SSSP, dg, o[1], 0, delta=delta))
-            # Symmetric graphs: the CSR is its own in-edge CSR.
-            ts.append(gdb.make_task(gdb.TASK_COLOR, dg, o[2], 0, node_value=nv, transpose=dg))
+            ts.append(gdb.make_task(gdb.TASK_COLOR, dg, o[2], 0, node_value=nv))
         return ts
 
     def run(wgs, groups, team, delta):
