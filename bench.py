@@ -246,10 +246,16 @@ def bench_c3(args) -> int:
         return st.kernel_ms, b, st
 
     if args.engine_only:
-        # Tuned kernels only, at the autotuner's shapes (profiler runs).
-        m_ms = mis(gdb.VARIANT_TUNED, 256, 2048)[0]
-        b_ms = bc(gdb.VARIANT_TUNED, 256, 2048)[0]
-        print(json.dumps({"mis_ms": round(m_ms, 3), "bc_ms": round(b_ms, 3)}), flush=True)
+        # Tuned kernels only, at the tuning table's shapes (profiler runs).
+        from gpu_discovery_barrier_amd import autotune
+
+        rm, rb = autotune.lookup("mis", g.n), autotune.lookup("bc", g.n)
+        sm = (rm["wgs"], rm["groups"]) if rm else (256, 2048)
+        sb = (rb["wgs"], rb["groups"]) if rb else (256, 2048)
+        m_ms = mis(gdb.VARIANT_TUNED, *sm)[0]
+        b_ms = bc(gdb.VARIANT_TUNED, *sb)[0]
+        print(json.dumps({"mis_ms": round(m_ms, 3), "bc_ms": round(b_ms, 3), "mis_shape": sm, "bc_shape": sb}),
+              flush=True)
         ctx.close()
         return 0
     shapes = [(w, gr) for w in (256, 512, 1024) for gr in (256, 512, 1024, 2048)
