@@ -38,3 +38,26 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout.strip() == ""
+
+
+def test_headline_engine_follows_the_tuning_table(tmp_path, monkeypatch):
+    # A batch_owned row (written by the autotuner only when the owned engine
+    # won) makes the owned engine the headline; without it the shared row is
+    # used; --engine forces either.
+    from gpu_discovery_barrier_amd import autotune
+
+    n = bench.GRID_SIDE * bench.GRID_SIDE
+    rows = [autotune.row("batch", 1 << 30, 128, 256, 128, 4096, 256, 2048),
+            autotune.row("batch_owned", 1 << 30, 1024, 256, 128, 8192, 0, 2048)]
+    table = tmp_path / "tune_table.txt"
+    autotune.write_table(rows, table)
+    assert autotune.lookup("batch_owned", n, table)["delta"] == 8192
+    monkeypatch.setattr(autotune, "TABLE", table)
+    monkeypatch.setattr(autotune, "lookup", lambda s, nn, path=table, _f=autotune.lookup: _f(s, nn, path))
+    shape, team, qcap, owned = bench.tuned_c2()
+    assert owned and shape == (1024, 256, 8192, 0, 0) and team == 128 and qcap == 2048
+    shape, team, qcap, owned = bench.tuned_c2("shared")
+    assert not owned and shape == (128, 256, 4096, 256, 0)
+    autotune.write_table(rows[:1], table)
+    assert bench.tuned_c2()[3] is False
+    assert bench.tuned_c2("owned")[3] is True
