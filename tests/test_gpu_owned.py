@@ -132,3 +132,28 @@ def test_owned_repeated_solves_reuse_state(ctx):
     for src in (0, 4321, 8999):
         got, _ = solve(ctx, g, BOTH, 32, delta=500, source=src)
         check(g, BOTH, got, source=src)
+
+
+def test_owned_batch_graph_replays_match_oracle(ctx):
+    # Owned BFS and SSSP captured as a HIP graph and replayed several times,
+    # with the outputs scrambled and eager work on the same context between
+    # replays: every replay recomputes exact levels and distances (the
+    # solve's ranges, marks and epoch counters are set up inside the kernel).
+    g = gdb.gen_grid(192, 160, 0.05, seed=12, wlo=1, whi=200)
+    dg = gdb.DeviceCSR(g)
+    outs = [gdb.DeviceBuffer.empty(g.n, np.uint32) for _ in BOTH]
+    tasks = [gdb.make_task(algo, dg, out, 0, delta=400 if algo == gdb.TASK_SSSP else 0, owned=True)
+             for algo, out in zip(BOTH, outs)]
+    graph = gdb.BatchGraph(ctx, tasks, 16, gdb.Opts.make(wgs=256, groups=32, lds_queue=1024))
+    try:
+        for rep in range(3):
+            for o in outs:
+                junk = np.full(o.count, 7, dtype=o.dtype)
+                gdb.check(gdb.lib().gdb_memcpy_htod(o.ptr, junk.ctypes.data_as(gdb.api.P), junk.nbytes), "htod")
+            graph.launch()
+            check(g, BOTH, [o.to_numpy() for o in outs])
+            small = gdb.gen_grid(40, 40, 0.0, seed=rep)
+            got, _ = solve(ctx, small, BOTH, 4, delta=100)  # eager work between replays
+            check(small, BOTH, got)
+    finally:
+        graph.close()
