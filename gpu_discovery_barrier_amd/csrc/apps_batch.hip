@@ -1039,6 +1039,21 @@ constexpr int kOwnedQueueMax = 2048;
 // Dynamic LDS of a batch with an owned task: a CU's 160 KiB less the kernel's
 // static LDS (a few hundred bytes).
 constexpr long long kOwnedLdsTotal = 163840 - 1024;
+// Slot records (device memory, task scratch after the boundary marks): each
+// range writes one record per vertex at the start of the solve, and a round
+// reads it in place of the row offsets and the CSR, one dependent load
+// instead of two.  A record holds up to kChunk out-edges (BFS: ids, 16 B;
+// SSSP: (id, weight) pairs, 32 B) padded with kSlotNone; a vertex with more
+// marks slot 0 with kSlotCsr and is walked from the CSR.
+constexpr int kSlotNone = -2147483647 - 1;
+constexpr int kSlotCsr = -2147483647;
+static_assert(kChunk == 4, "a slot record is one or two 16-byte vectors");
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) i32x4 g_i32x4;
+// Scratch ints of an owned task: boundary marks, then the slot records.
+__host__ __device__ inline size_t owned_scratch_ints(int n, bool unit) {
+  return ((((size_t)n + 31) / 32 + 63) & ~(size_t)63) + (size_t)n * kChunk * (unit ? 1 : 2);
+}
 
 // Bytes of LDS the owned solve needs without the optional parts.
 __host__ __device__ constexpr long long owned_lds_core(long long B, int qcap) {
@@ -1094,6 +1109,8 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
   // alias.
   g_u32* const out = (g_u32*)k.out;
   g_u32* const bmark = (g_u32*)k.scratch;
+  g_i32x4* const slots = (g_i32x4*)(k.scratch + ((((size_t)n + 31) / 32 + 63) & ~(size_t)63));
+  constexpr int kRec = UNIT ? 1 : 2;  // 16-byte vectors per slot record
   g_ci32* const row = (g_ci32*)k.row;
   g_ci32* const col = (g_ci32*)k.col;
   g_ci32* const wgt = (g_ci32*)k.w;
@@ -1144,6 +1161,44 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
         },
         kSerialLimit);
   }
+  // Slot records of this range (only this group reads them; the barrier
+  // below orders the stores before the first round).
+  {
+    constexpr int kU = 4;  // vertices per thread per pass, their loads in flight together
+    for (int base = 0; base < nb; base += kU * (int)blockDim.x) {
+      int b[kU], d[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int i = base + u * (int)blockDim.x + (int)threadIdx.x;
+        b[u] = 0;
+        d[u] = 0;
+        if (i < nb) {
+          b[u] = row[lo + i];
+          d[u] = row[lo + i + 1] - b[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int i = base + u * (int)blockDim.x + (int)threadIdx.x;
+        int c[kChunk], x[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const bool in = j < d[u] && d[u] <= kChunk;
+          c[j] = in ? col[b[u] + j] : kSlotNone;
+          x[j] = (in && !UNIT) ? wgt[b[u] + j] : 0;
+        }
+        if (d[u] > kChunk) c[0] = kSlotCsr;
+        if (i < nb) {
+          if (UNIT) {
+            slots[lo + i] = i32x4{c[0], c[1], c[2], c[3]};
+          } else {
+            slots[2 * (size_t)(lo + i)] = i32x4{c[0], x[0], c[1], x[1]};
+            slots[2 * (size_t)(lo + i) + 1] = i32x4{c[2], x[2], c[3], x[3]};
+          }
+        }
+      }
+    }
+  }
   if ((unsigned)(source - lo) < (unsigned)nb && threadIdx.x == 0) {
     const int s0 = source - lo;
     dist[s0] = 0;
@@ -1176,42 +1231,64 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
   const int kScanEvery = UNIT ? kOwnedScanEveryBfs : rowl ? kOwnedScanEverySsspRows : kOwnedScanEverySssp;
   unsigned thresh = band;
   long long items = 0, edges = 0;
-  // Relaxes local vertex li to cand on lanes with act.  An improvement below
-  // the threshold joins queue buffer qb (once: its inq bit); one above it, or
-  // one that finds the buffer full, joins the far set.  All lanes of the wave
-  // call it.  Returns whether this lane improved.
-  auto relax_local = [&](bool act, int li, unsigned cand, int qb, int cb) -> bool {
-    bool imp = false, push = false;
-    const unsigned bit = 1u << (li & 31);
-    if (act) {
-      imp = cand < __hip_atomic_fetch_min(&dist[li], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (imp) {
-        if (cand < thresh) push = !(lds_or(&inq[li >> 5], bit) & bit);
-        else lds_or(&farb[li >> 5], bit);
+  // Relaxes local vertices li[j] to cand[j] on lanes with act[j], N targets
+  // per lane together: the N LDS minima are issued back to back, then the
+  // marks, then one queue reservation for the wave (three LDS round trips
+  // instead of 3N).  An improvement below the threshold joins queue buffer
+  // qb (once: its inq bit); one above it, or one that finds the buffer full,
+  // joins the far set.  All lanes of the wave call it.  imp[j]: target j of
+  // this lane improved.
+  auto relax_many = [&](const auto& act, const auto& li, const auto& cand, auto& imp, int qb, int cb) {
+    constexpr int N = (int)(sizeof(act) / sizeof(act[0]));
+    unsigned old[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      old[j] = act[j] ? __hip_atomic_fetch_min(&dist[li[j]], cand[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                      : 0u;
+    unsigned prev[N];  // the inq word before this lane's mark (the bit itself when nothing is to be queued)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      imp[j] = act[j] && cand[j] < old[j];
+      const unsigned bit = 1u << (li[j] & 31);
+      prev[j] = bit;
+      if (imp[j]) {
+        if (cand[j] < thresh) prev[j] = lds_or(&inq[li[j] >> 5], bit);
+        else lds_or(&farb[li[j] >> 5], bit);
       }
     }
-    const unsigned long long m = __ballot(push);
-    if (m) {
-      const int leader = __ffsll((long long)m) - 1;
+    unsigned long long m[N], all = 0;
+    int total = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      m[j] = __ballot(!(prev[j] & (1u << (li[j] & 31))));
+      all |= m[j];
+      total += __popcll(m[j]);
+    }
+    if (all) {
+      const int leader = __ffsll((long long)all) - 1;
       int base = 0;
-      if (lane == leader) base = atomicAdd(&s_cnt[cb], __popcll(m));
+      if (lane == leader) base = atomicAdd(&s_cnt[cb], total);
       base = gdb::bcast(base, leader);
-      if (push) {
-        const int pos = base + __popcll(m & lt);
-        if (pos < qc) {
-          q[qb * qc + pos] = (unsigned short)li;
-        } else {
-          lds_and(&inq[li >> 5], ~bit);
-          lds_or(&farb[li >> 5], bit);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        if ((m[j] >> lane) & 1ull) {
+          const int pos = base + __popcll(m[j] & lt);
+          const unsigned bit = 1u << (li[j] & 31);
+          if (pos < qc) {
+            q[qb * qc + pos] = (unsigned short)li[j];
+          } else {
+            lds_and(&inq[li[j] >> 5], ~bit);
+            lds_or(&farb[li[j] >> 5], bit);
+          }
         }
+        base += __popcll(m[j]);
       }
     }
-    return imp;
   };
   // Relaxes every boundary vertex from its output word, kScanBatch loads per
   // thread in flight; sets s_flag when an improvement joined the queue
   // (below the threshold).
-  constexpr int kScanBatch = 16;
+  constexpr int kScanBatch = 16, kScanGroup = 4;
   auto scan = [&](int qb, int cb) {
     bool any = false;
     const int step = kScanBatch * (int)blockDim.x;
@@ -1225,9 +1302,22 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
         x[j] = li[j] >= 0 ? __hip_atomic_load(&out[lo + li[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                           : kInf;
       }
+      // Relaxed kScanGroup at a time (register pressure).
 #pragma unroll
-      for (int j = 0; j < kScanBatch; ++j)
-        any |= relax_local(x[j] < kInf, li[j] < 0 ? 0 : li[j], x[j], qb, cb) && x[j] < thresh;
+      for (int h = 0; h < kScanBatch; h += kScanGroup) {
+        bool act[kScanGroup], imp[kScanGroup];
+        int lg[kScanGroup];
+        unsigned xg[kScanGroup];
+#pragma unroll
+        for (int j = 0; j < kScanGroup; ++j) {
+          act[j] = x[h + j] < kInf;
+          lg[j] = li[h + j] < 0 ? 0 : li[h + j];
+          xg[j] = x[h + j];
+        }
+        relax_many(act, lg, xg, imp, qb, cb);
+#pragma unroll
+        for (int j = 0; j < kScanGroup; ++j) any |= imp[j] && xg[j] < thresh;
+      }
     }
     if (__ballot(any) && lane == 0) s_flag = 1;
   };
@@ -1237,6 +1327,8 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
     GDB_FOR_UNIFORM(i, valid, (int)threadIdx.x, (int)blockDim.x, ln) {
       int li = 0, b = 0, e = 0;
       unsigned du = kInf;
+      i32x4 rec[kRec];
+      bool csr = false;
       if (valid) {
         li = q[qa * qc + i];
         // Clear the mark before reading the distance: a later improvement
@@ -1246,10 +1338,48 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
         lds_and(&inq[li >> 5], ~(1u << (li & 31)));
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         du = __hip_atomic_load(&dist[li], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        b = rowl ? rowl[li] : row[lo + li];
-        e = rowl ? rowl[li + 1] : row[lo + li + 1];
+#pragma unroll
+        for (int h = 0; h < kRec; ++h) rec[h] = slots[kRec * (size_t)(lo + li) + h];
+        csr = rec[0].x == kSlotCsr;
+        if (csr) {
+          b = rowl ? rowl[li] : row[lo + li];
+          e = rowl ? rowl[li + 1] : row[lo + li + 1];
+        }
         ++items;
-        edges += e - b;
+      } else {
+#pragma unroll
+        for (int h = 0; h < kRec; ++h) rec[h] = i32x4{kSlotNone, 0, kSlotNone, 0};
+      }
+      // Relaxes up to kChunk targets v (weights wt) of a vertex at src_d.
+      auto relax_targets = [&](const bool (&ok)[kChunk], const int (&v)[kChunk], const unsigned (&wt)[kChunk],
+                               unsigned src_d) {
+        bool act[kChunk], imp[kChunk];
+        int li[kChunk];
+        unsigned cand[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const bool loc = (unsigned)(v[j] - lo) < (unsigned)nb;
+          const bool rem = !loc && (unsigned)v[j] < (unsigned)n;
+          cand[j] = src_d + wt[j];
+          if (ok[j] && rem && wt[j] < kInf)
+            __hip_atomic_fetch_min(&out[v[j]], cand[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          act[j] = ok[j] && loc && wt[j] < kInf;
+          li[j] = loc ? v[j] - lo : 0;
+        }
+        relax_many(act, li, cand, imp, qb, cb);
+      };
+      {
+        bool ok[kChunk];
+        int v[kChunk];
+        unsigned wt[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          v[j] = UNIT ? rec[0][j] : rec[j >> 1][2 * (j & 1)];
+          wt[j] = UNIT ? 1u : (unsigned)rec[j >> 1][2 * (j & 1) + 1];
+          ok[j] = valid && !csr && v[j] != kSlotNone;
+          edges += ok[j];
+        }
+        relax_targets(ok, v, wt, du);
       }
       auto relax_edges = [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], unsigned src_d) {
         int v[kChunk];
@@ -1260,18 +1390,11 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
           v[j] = ok[j] ? col[ee] : -1;
           wt[j] = UNIT ? 1u : (ok[j] ? (unsigned)wgt[ee] : kInf);
         }
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-          const bool loc = (unsigned)(v[j] - lo) < (unsigned)nb;
-          const bool rem = !loc && (unsigned)v[j] < (unsigned)n;
-          const unsigned cand = src_d + wt[j];
-          if (ok[j] && rem && wt[j] < kInf)
-            __hip_atomic_fetch_min(&out[v[j]], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          relax_local(ok[j] && loc && wt[j] < kInf, loc ? v[j] - lo : 0, cand, qb, cb);
-        }
+        relax_targets(ok, v, wt, src_d);
       };
-      const bool heavy = valid && (e - b) > kSerialLimit;
-      const int len = (valid && !heavy) ? (e - b) : 0;
+      if (csr) edges += e - b;
+      const bool heavy = csr && (e - b) > kSerialLimit;
+      const int len = (csr && !heavy) ? (e - b) : 0;
       const int maxlen = gdb::wave_max(len);
       for (int c = 0; c < maxlen; c += kChunk) {
         bool ok[kChunk];
@@ -1604,6 +1727,7 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
   // Per-task scratch sizes (ints) and the device task table.
   std::vector<DevTask> dt((size_t)ntasks);
+  std::vector<size_t> sints((size_t)ntasks);
   size_t scratch_total = 0;
   for (int t = 0; t < ntasks; ++t) {
     const gdb_task& a = tasks[order[t]];
@@ -1615,11 +1739,13 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
         (cfg.mode != GDB_SEL_NONE && a.algo != GDB_TASK_COLOR && a.n > kNinMask))
       return GDB_E_INVALID;
     size_t ints = (size_t)(a.algo == GDB_TASK_SSSP || a.hops > 0 ? 9 : a.algo == GDB_TASK_COLOR ? 5 : 2) * a.n;
-    // An owned solve keeps its boundary marks (a bit per vertex) there.
-    if (a.owned && a.algo != GDB_TASK_COLOR) ints = std::max(ints, ((size_t)a.n + 31) / 32);
+    // An owned solve keeps its boundary marks (a bit per vertex) and slot
+    // records there (or runs the shared-queue engine, which needs `ints`).
+    if (a.owned && a.algo != GDB_TASK_COLOR) ints = std::max(ints, owned_scratch_ints(a.n, a.algo == GDB_TASK_BFS));
     dt[t] = DevTask{a.algo, a.n, a.m, a.source, a.delta, a.hops, a.owned, a.row, a.col, a.w, a.node_value, a.out,
                     nullptr, a.trow, a.tcol, a.tw};
-    scratch_total += (ints + 63) & ~(size_t)63;
+    sints[t] = (ints + 63) & ~(size_t)63;
+    scratch_total += sints[t];
   }
   // Only local hops and owned solves use the LDS queue: without them it
   // takes no LDS (more groups fit a CU).  An owned solve takes what a CU
@@ -1668,10 +1794,8 @@ int plan_batch(gdb_ctx* ctx, const gdb_task* tasks, int ntasks, int team_groups,
   int* scratch = cv.take<int>(scratch_total);
   size_t off = 0;
   for (int t = 0; t < ntasks; ++t) {
-    const size_t ints =
-        (size_t)(dt[t].algo == GDB_TASK_SSSP || dt[t].hops > 0 ? 9 : dt[t].algo == GDB_TASK_COLOR ? 5 : 2) * dt[t].n;
     dt[t].scratch = scratch + off;
-    off += (ints + 63) & ~(size_t)63;
+    off += sints[t];
   }
   GDB_HIP(hipMemcpyAsync(P->d_tasks, dt.data(), sizeof(DevTask) * ntasks, hipMemcpyHostToDevice, ctx->stream));
   GDB_HIP(hipStreamSynchronize(ctx->stream));  // dt is a local copy
