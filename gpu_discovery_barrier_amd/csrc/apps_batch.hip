@@ -992,18 +992,19 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
 
 // ---- Owner-partitioned BFS / SSSP on a team (gdb_task.owned) --------------
 // Group r of a team of G owns the vertex range [r*B, r*B + B), B = ceil(n/G)
-// rounded up to 32, and keeps the range's distances, row offsets, queue
-// marks and queues in LDS.  A relaxation whose destination lies in the range
-// is an LDS atomic, and the group's rounds are separated by workgroup
-// barriers only, so a hop along a path inside a range costs one memory round
-// trip (the edge loads) instead of three.  A relaxation into another range is
-// a fire-and-forget device-scope atomic min on the destination's output
-// word; an owner reads the output words of its boundary vertices (the ones
-// with an in-edge from another range, marked once per task) every few
-// rounds (a scan costs about 3 us: one device-scope load per boundary
-// vertex) and while it is idle.  (A per-owner counter of such updates, to
-// skip scans that find nothing, measured slower on a 2048^2 grid: every
-// scan there finds something, and the extra atomic per edge costs.)
+// rounded up to 32, and keeps the range's distances, queue marks and queues
+// (and row offsets, when they fit) in LDS.  A relaxation whose destination
+// lies in the range is an LDS atomic, and the group's rounds are separated by
+// workgroup barriers only, so a hop along a path inside a range costs one
+// memory round trip (the vertex's slot record, below) instead of three.  A
+// relaxation into another range is a fire-and-forget device-scope atomic min
+// on the destination's output word; an owner reads the output words of its
+// boundary vertices (the ones with an in-edge from another range, marked
+// once per task) every few rounds (one device-scope load per boundary vertex,
+// up to kScanBatch in flight per thread: 3.3-3.7 us per scan on the grid)
+// and while it is idle.  (A per-owner counter of such updates, to skip scans that find
+// nothing, measured slower on a 2048^2 grid: every scan there finds
+// something, and the extra atomic per edge costs.)
 //
 // Distance bands: the team relaxes only below a common threshold T (SSSP:
 // delta; BFS: kOwnedBfsBand levels); an improvement at or above T waits in
@@ -1027,11 +1028,13 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
 // queue u16[2][qcap] | row offsets[B + 1] when they fit | boundary list u16.
 // Rounds between boundary scans of a busy group, and BFS levels per band,
 // as measured on the 2048^2 grid (scans every 2 / 4 / 8 / 16 rounds, bands
-// of 32 to 512 levels): BFS 8; SSSP 4 when the row offsets are in LDS (256
-// ranges: 19.2 against 20.7 ms at 8), 8 when they are not (128 ranges: 19.5
-// against 20.2 ms at 4).
+// of 128 to 512 levels, with slot records and batched relaxations): 8 for
+// both (128 ranges; BFS 10.3 ms at 8, 11.3 at 4; SSSP 14.2 ms at 8, 14.9 at
+// 4, 14.8 at 16); 4 for SSSP when the row offsets are in LDS (256 ranges,
+// before slot records: 19.2 against 20.7 ms at 8).  BFS bands of 512 levels
+// (9.35 ms, against 9.7 at 256 and 11.1 at 128).
 constexpr int kOwnedScanEveryBfs = 8, kOwnedScanEverySsspRows = 4, kOwnedScanEverySssp = 8;
-constexpr int kOwnedBfsBand = 256;
+constexpr int kOwnedBfsBand = 512;
 // Queue entries per buffer at most: a larger tile would take the LDS the
 // boundary list needs (a range whose list does not fit scans all of its
 // vertices instead: 23.8 against 19.5 ms at a 4096-entry tile).
@@ -1288,42 +1291,58 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
   // Relaxes every boundary vertex from its output word, kScanBatch loads per
   // thread in flight; sets s_flag when an improvement joined the queue
   // (below the threshold).
+  // A thread's boundary entries are base + j * blockDim.x + threadIdx.x;
+  // scan_load loads kScanGroup of them, scan_relax relaxes them (and says
+  // whether one improved below the threshold).
   constexpr int kScanBatch = 16, kScanGroup = 4;
-  auto scan = [&](int qb, int cb) {
+  auto scan_load = [&](int base, int (&li)[kScanGroup], unsigned (&x)[kScanGroup]) {
+#pragma unroll
+    for (int j = 0; j < kScanGroup; ++j) {
+      const int i = base + j * (int)blockDim.x + (int)threadIdx.x;
+      li[j] = i < nscan ? (scan_all ? i : (int)bl[i]) : -1;
+      x[j] = li[j] >= 0 ? __hip_atomic_load(&out[lo + li[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kInf;
+    }
+  };
+  auto scan_relax = [&](const int (&li)[kScanGroup], const unsigned (&x)[kScanGroup], int qb, int cb) -> bool {
+    bool act[kScanGroup], imp[kScanGroup];
+    int lg[kScanGroup];
+#pragma unroll
+    for (int j = 0; j < kScanGroup; ++j) {
+      act[j] = x[j] < kInf;
+      lg[j] = li[j] < 0 ? 0 : li[j];
+    }
+    relax_many(act, lg, x, imp, qb, cb);
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < kScanGroup; ++j) any |= imp[j] && x[j] < thresh;
+    return any;
+  };
+  // Boundary entries from `start` on, kScanBatch loads per thread in flight
+  // (relaxed kScanGroup at a time: register pressure).
+  auto scan = [&](int qb, int cb, int start) {
     bool any = false;
     const int step = kScanBatch * (int)blockDim.x;
-    for (int base = 0; base < nscan; base += step) {
-      int li[kScanBatch];
-      unsigned x[kScanBatch];
+    for (int base = start; base < nscan; base += step) {
+      int li[kScanBatch / kScanGroup][kScanGroup];
+      unsigned x[kScanBatch / kScanGroup][kScanGroup];
 #pragma unroll
-      for (int j = 0; j < kScanBatch; ++j) {
-        const int i = base + j * (int)blockDim.x + (int)threadIdx.x;
-        li[j] = i < nscan ? (scan_all ? i : (int)bl[i]) : -1;
-        x[j] = li[j] >= 0 ? __hip_atomic_load(&out[lo + li[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                          : kInf;
-      }
-      // Relaxed kScanGroup at a time (register pressure).
+      for (int h = 0; h < kScanBatch / kScanGroup; ++h) scan_load(base + h * kScanGroup * (int)blockDim.x, li[h], x[h]);
 #pragma unroll
-      for (int h = 0; h < kScanBatch; h += kScanGroup) {
-        bool act[kScanGroup], imp[kScanGroup];
-        int lg[kScanGroup];
-        unsigned xg[kScanGroup];
-#pragma unroll
-        for (int j = 0; j < kScanGroup; ++j) {
-          act[j] = x[h + j] < kInf;
-          lg[j] = li[h + j] < 0 ? 0 : li[h + j];
-          xg[j] = x[h + j];
-        }
-        relax_many(act, lg, xg, imp, qb, cb);
-#pragma unroll
-        for (int j = 0; j < kScanGroup; ++j) any |= imp[j] && xg[j] < thresh;
-      }
+      for (int h = 0; h < kScanBatch / kScanGroup; ++h) any |= scan_relax(li[h], x[h], qb, cb);
     }
     if (__ballot(any) && lane == 0) s_flag = 1;
   };
   // Relaxes the out-edges of queue buffer (qa, count ca) into (qb, cb).
-  auto round = [&](int qa, int ca, int qb, int cb) {
+  // with_scan: then a boundary scan, whose first kScanGroup loads per thread
+  // are issued with the round's first slot-record load and relaxed after the
+  // round's edges (their latency overlaps the round's).  BFS only: on the
+  // 2048^2 grid this took BFS from 10.3 to 9.7 ms and SSSP from 14.0 to
+  // 14.2 ms (same box; SSSP's rounds hold more registers).
+  auto round = [&](int qa, int ca, int qb, int cb, bool with_scan) {
     const int ln = min(s_cnt[ca], qc);
+    int sl[kScanGroup];
+    unsigned sx[kScanGroup];
+    bool issued = false;  // uniform in the wave
     GDB_FOR_UNIFORM(i, valid, (int)threadIdx.x, (int)blockDim.x, ln) {
       int li = 0, b = 0, e = 0;
       unsigned du = kInf;
@@ -1340,15 +1359,21 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
         du = __hip_atomic_load(&dist[li], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
         for (int h = 0; h < kRec; ++h) rec[h] = slots[kRec * (size_t)(lo + li) + h];
+      } else {
+#pragma unroll
+        for (int h = 0; h < kRec; ++h) rec[h] = i32x4{kSlotNone, 0, kSlotNone, 0};
+      }
+      if (with_scan && !issued) {
+        scan_load(0, sl, sx);
+        issued = true;
+      }
+      if (valid) {
         csr = rec[0].x == kSlotCsr;
         if (csr) {
           b = rowl ? rowl[li] : row[lo + li];
           e = rowl ? rowl[li + 1] : row[lo + li + 1];
         }
         ++items;
-      } else {
-#pragma unroll
-        for (int h = 0; h < kRec; ++h) rec[h] = i32x4{kSlotNone, 0, kSlotNone, 0};
       }
       // Relaxes up to kChunk targets v (weights wt) of a vertex at src_d.
       auto relax_targets = [&](const bool (&ok)[kChunk], const int (&v)[kChunk], const unsigned (&wt)[kChunk],
@@ -1420,6 +1445,11 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
           },
           -1);
     }
+    if (with_scan) {
+      if (!issued) scan_load(0, sl, sx);
+      scan_relax(sl, sx, qb, cb);
+      scan(qb, cb, kScanGroup * (int)blockDim.x);
+    }
   };
   // Smallest distance in the far set (kInf when empty), in s_min.
   auto far_min = [&]() {
@@ -1480,8 +1510,9 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
       if (s_cnt[ca] == 0) break;
       const int cb = (s + 1) % 3, qa = s & 1, qb = qa ^ 1;
       if (threadIdx.x == 0) s_cnt[(s + 2) % 3] = 0;
-      round(qa, ca, qb, cb);
-      if (s % kScanEvery == kScanEvery - 1) scan(qb, cb);
+      const bool due = s % kScanEvery == kScanEvery - 1;
+      round(qa, ca, qb, cb, UNIT && due);
+      if (!UNIT && due) scan(qb, cb, 0);
       ++s;
       __syncthreads();
     }
@@ -1517,7 +1548,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
       __syncthreads();
       if (T.l->aborted) return -1;
       if (s_c == G) break;
-      scan(s & 1, s % 3);
+      scan(s & 1, s % 3, 0);
       __syncthreads();
       if (s_flag) {
         if (threadIdx.x == 0) {
@@ -1544,7 +1575,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
     if (!pending) {
       if (threadIdx.x == 0) s_flag = 0;
       __syncthreads();
-      scan(s & 1, s % 3);
+      scan(s & 1, s % 3, 0);
       __syncthreads();
       pending = s_flag != 0;
     }
