@@ -1291,14 +1291,15 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
   // Relaxes every boundary vertex from its output word, kScanBatch loads per
   // thread in flight; sets s_flag when an improvement joined the queue
   // (below the threshold).
-  // A thread's boundary entries are base + j * blockDim.x + threadIdx.x;
-  // scan_load loads kScanGroup of them, scan_relax relaxes them (and says
-  // whether one improved below the threshold).
-  constexpr int kScanBatch = 16, kScanGroup = 4;
-  auto scan_load = [&](int base, int (&li)[kScanGroup], unsigned (&x)[kScanGroup]) {
+  // The scan runs on nt threads (t = 0 .. nt-1 among them); thread t's
+  // boundary entries are base + j * nt + t.  scan_load loads kScanGroup of
+  // them, scan_relax relaxes them (and says whether one improved below the
+  // threshold).
+  constexpr int kScanBatch = 8, kScanGroup = 4;  // 16 in flight spilled registers into the rounds
+  auto scan_load = [&](int base, int t, int nt, int (&li)[kScanGroup], unsigned (&x)[kScanGroup]) {
 #pragma unroll
     for (int j = 0; j < kScanGroup; ++j) {
-      const int i = base + j * (int)blockDim.x + (int)threadIdx.x;
+      const int i = base + j * nt + t;
       li[j] = i < nscan ? (scan_all ? i : (int)bl[i]) : -1;
       x[j] = li[j] >= 0 ? __hip_atomic_load(&out[lo + li[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kInf;
     }
@@ -1317,32 +1318,24 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
     for (int j = 0; j < kScanGroup; ++j) any |= imp[j] && x[j] < thresh;
     return any;
   };
-  // Boundary entries from `start` on, kScanBatch loads per thread in flight
-  // (relaxed kScanGroup at a time: register pressure).
-  auto scan = [&](int qb, int cb, int start) {
+  // Every boundary entry, kScanBatch loads per thread in flight (relaxed
+  // kScanGroup at a time: register pressure).
+  auto scan = [&](int qb, int cb, int t, int nt) {
     bool any = false;
-    const int step = kScanBatch * (int)blockDim.x;
-    for (int base = start; base < nscan; base += step) {
+    const int step = kScanBatch * nt;
+    for (int base = 0; base < nscan; base += step) {
       int li[kScanBatch / kScanGroup][kScanGroup];
       unsigned x[kScanBatch / kScanGroup][kScanGroup];
 #pragma unroll
-      for (int h = 0; h < kScanBatch / kScanGroup; ++h) scan_load(base + h * kScanGroup * (int)blockDim.x, li[h], x[h]);
+      for (int h = 0; h < kScanBatch / kScanGroup; ++h) scan_load(base + h * kScanGroup * nt, t, nt, li[h], x[h]);
 #pragma unroll
       for (int h = 0; h < kScanBatch / kScanGroup; ++h) any |= scan_relax(li[h], x[h], qb, cb);
     }
     if (__ballot(any) && lane == 0) s_flag = 1;
   };
   // Relaxes the out-edges of queue buffer (qa, count ca) into (qb, cb).
-  // with_scan: then a boundary scan, whose first kScanGroup loads per thread
-  // are issued with the round's first slot-record load and relaxed after the
-  // round's edges (their latency overlaps the round's).  BFS only: on the
-  // 2048^2 grid this took BFS from 10.3 to 9.7 ms and SSSP from 14.0 to
-  // 14.2 ms (same box; SSSP's rounds hold more registers).
-  auto round = [&](int qa, int ca, int qb, int cb, bool with_scan) {
+  auto round = [&](int qa, int ca, int qb, int cb) {
     const int ln = min(s_cnt[ca], qc);
-    int sl[kScanGroup];
-    unsigned sx[kScanGroup];
-    bool issued = false;  // uniform in the wave
     GDB_FOR_UNIFORM(i, valid, (int)threadIdx.x, (int)blockDim.x, ln) {
       int li = 0, b = 0, e = 0;
       unsigned du = kInf;
@@ -1362,10 +1355,6 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
       } else {
 #pragma unroll
         for (int h = 0; h < kRec; ++h) rec[h] = i32x4{kSlotNone, 0, kSlotNone, 0};
-      }
-      if (with_scan && !issued) {
-        scan_load(0, sl, sx);
-        issued = true;
       }
       if (valid) {
         csr = rec[0].x == kSlotCsr;
@@ -1445,11 +1434,6 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
           },
           -1);
     }
-    if (with_scan) {
-      if (!issued) scan_load(0, sl, sx);
-      scan_relax(sl, sx, qb, cb);
-      scan(qb, cb, kScanGroup * (int)blockDim.x);
-    }
   };
   // Smallest distance in the far set (kInf when empty), in s_min.
   auto far_min = [&]() {
@@ -1510,9 +1494,17 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
       if (s_cnt[ca] == 0) break;
       const int cb = (s + 1) % 3, qa = s & 1, qb = qa ^ 1;
       if (threadIdx.x == 0) s_cnt[(s + 2) % 3] = 0;
+      // A boundary scan every kScanEvery rounds.  When the queue leaves
+      // waves without an entry, those waves scan while the others run the
+      // round (the two overlap); otherwise the whole group scans after it.
+      const int split = (min(s_cnt[ca], qc) + 63) & ~63;  // threads with an entry, in whole 64-lane waves
       const bool due = s % kScanEvery == kScanEvery - 1;
-      round(qa, ca, qb, cb, UNIT && due);
-      if (!UNIT && due) scan(qb, cb, 0);
+      if (due && split < (int)blockDim.x && (int)threadIdx.x >= split) {
+        scan(qb, cb, (int)threadIdx.x - split, (int)blockDim.x - split);
+      } else {
+        round(qa, ca, qb, cb);
+        if (due && split >= (int)blockDim.x) scan(qb, cb, (int)threadIdx.x, (int)blockDim.x);
+      }
       ++s;
       __syncthreads();
     }
@@ -1548,7 +1540,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
       __syncthreads();
       if (T.l->aborted) return -1;
       if (s_c == G) break;
-      scan(s & 1, s % 3, 0);
+      scan(s & 1, s % 3, (int)threadIdx.x, (int)blockDim.x);
       __syncthreads();
       if (s_flag) {
         if (threadIdx.x == 0) {
@@ -1575,7 +1567,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
     if (!pending) {
       if (threadIdx.x == 0) s_flag = 0;
       __syncthreads();
-      scan(s & 1, s % 3, 0);
+      scan(s & 1, s % 3, (int)threadIdx.x, (int)blockDim.x);
       __syncthreads();
       pending = s_flag != 0;
     }
