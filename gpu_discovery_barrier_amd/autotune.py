@@ -176,7 +176,7 @@ def tune_batch_grid(ctx, quick=False):
     otrials = []
     for w2 in ((1024,) if quick else (512, 1024)):
         gr2 = api.max_resident_groups(w2, OWNED_LDS)  # one group per CU
-        for dl in ((8192,) if quick else (6144, 8192, 12288, 16384)):
+        for dl in ((6144,) if quick else (4096, 6144, 8192, 12288)):
             ms2 = min(grid_batch_run(ctx, dg, ob, os_, w2, gr2, gr2 // 2, dl, 0, 2048, owned=True) for _ in range(2))
             otrials.append((ms2, w2, gr2, gr2 // 2, dl, 0, 2048))
     oms, ow, ogr, oteam, odl, _, oq = best_of(otrials)

@@ -1027,14 +1027,13 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
 // LDS (dynamic, `region` bytes): dist[B] u32 | inq[B/32] | far[B/32] |
 // queue u16[2][qcap] | row offsets[B + 1] when they fit | boundary list u16.
 // Rounds between boundary scans of a busy group, and BFS levels per band,
-// as measured on the 2048^2 grid (scans every 2 / 4 / 8 / 16 rounds, bands
-// of 128 to 512 levels, with slot records and batched relaxations): 8 for
-// both (128 ranges; BFS 10.3 ms at 8, 11.3 at 4; SSSP 14.2 ms at 8, 14.9 at
-// 4, 14.8 at 16); 4 for SSSP when the row offsets are in LDS (256 ranges,
-// before slot records: 19.2 against 20.7 ms at 8).  BFS bands of 512 levels
-// (9.35 ms, against 9.7 at 256 and 11.1 at 128).
-constexpr int kOwnedScanEveryBfs = 8, kOwnedScanEverySsspRows = 4, kOwnedScanEverySssp = 8;
-constexpr int kOwnedBfsBand = 512;
+// as measured on the 2048^2 grid (128 ranges; scans every 2 / 4 / 6 / 8
+// rounds, bands of 512 and 1024 levels; scans beside the rounds): 4 for both
+// (SSSP 12.8 ms at 4, 13.1 at 6, 13.3 at 8, 13.5+ at 2; BFS 8.65 ms at 4
+// with 1024-level bands, 8.8 with 512).  When the scan ran after the round
+// instead, 8 was best (SSSP 14.2 ms against 14.9 at 4).
+constexpr int kOwnedScanEveryBfs = 4, kOwnedScanEverySsspRows = 4, kOwnedScanEverySssp = 4;
+constexpr int kOwnedBfsBand = 1024;
 // Queue entries per buffer at most: a larger tile would take the LDS the
 // boundary list needs (a range whose list does not fit scans all of its
 // vertices instead: 23.8 against 19.5 ms at a 4096-entry tile).
