@@ -61,6 +61,40 @@ def test_owned_rmat_cross_range_edges_exact(ctx, team):
     check(g, BOTH, got, source=3)
 
 
+def slot_edge_cases(n=3000, seed=21):
+    # Out-degrees 0..7 (slot records with 0 to 4 edges, and the CSR walk from
+    # 5 edges on), one vertex of degree 40 (walked by the whole wave), a
+    # parallel edge in every record of 2+ edges (the vertex's first target
+    # again, lighter, so a lane relaxes one vertex twice with two candidates),
+    # a self-loop on every fifth vertex of degree 3+, and weights of 1.2e9 on
+    # every 50th edge (at or above 1e9: they never improve a distance, in the
+    # oracle as in the kernel).
+    rng = np.random.default_rng(seed)
+    deg = (np.arange(n) % 8).astype(np.int64)
+    deg[17] = 40
+    row = np.zeros(n + 1, np.int64)
+    row[1:] = np.cumsum(deg)
+    col = rng.integers(0, n, int(row[-1])).astype(np.int32)
+    w = rng.integers(100, 201, col.size).astype(np.int32)
+    for v in range(n):
+        b, d = int(row[v]), int(deg[v])
+        if d >= 2:
+            col[b + 1] = col[b]
+            w[b + 1] = rng.integers(1, 100)
+        if d >= 3 and v % 5 == 0:
+            col[b + 2] = v
+    w[::50] = 1_200_000_000
+    return gdb.CSR(n, row.astype(np.int32), col, w)
+
+
+@pytest.mark.parametrize("team,wgs", [(1, 256), (4, 128), (16, 64)])
+def test_owned_slot_record_edge_cases(ctx, team, wgs):
+    g = slot_edge_cases()
+    for source in (1, 17, 2999):
+        got, _ = solve(ctx, g, BOTH, team, wgs=wgs, delta=150, source=source)
+        check(g, BOTH, got, source=source)
+
+
 def test_owned_queue_overflow_goes_through_far_set(ctx):
     # One group owns all of an R-MAT 13 graph with a 256-entry queue: the
     # hub's frontier overflows the queue into the far set.
