@@ -1051,7 +1051,9 @@ constexpr int kSlotNone = -2147483647 - 1;
 constexpr int kSlotCsr = -2147483647;
 static_assert(kChunk == 4, "a slot record is one or two 16-byte vectors");
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) i32x4 g_i32x4;
+typedef __attribute__((address_space(1))) const i32x2 g_ci32x2;
 // Scratch ints of an owned task: boundary marks, then the slot records.
 __host__ __device__ inline size_t owned_scratch_ints(int n, bool unit) {
   return ((((size_t)n + 31) / 32 + 63) & ~(size_t)63) + (size_t)n * kChunk * (unit ? 1 : 2);
@@ -1112,7 +1114,6 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
   g_u32* const out = (g_u32*)k.out;
   g_u32* const bmark = (g_u32*)k.scratch;
   g_i32x4* const slots = (g_i32x4*)(k.scratch + ((((size_t)n + 31) / 32 + 63) & ~(size_t)63));
-  constexpr int kRec = UNIT ? 1 : 2;  // 16-byte vectors per slot record
   g_ci32* const row = (g_ci32*)k.row;
   g_ci32* const col = (g_ci32*)k.col;
   g_ci32* const wgt = (g_ci32*)k.w;
@@ -1333,35 +1334,54 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
     if (__ballot(any) && lane == 0) s_flag = 1;
   };
   // Relaxes the out-edges of queue buffer (qa, count ca) into (qb, cb).
+  // Four lanes per queued vertex, one slot-record edge each: a round is
+  // usually run by one or two waves, so its length is their instruction
+  // count.  The first lane of a vertex marked kSlotCsr walks its CSR row.
   auto round = [&](int qa, int ca, int qb, int cb) {
     const int ln = min(s_cnt[ca], qc);
-    GDB_FOR_UNIFORM(i, valid, (int)threadIdx.x, (int)blockDim.x, ln) {
-      int li = 0, b = 0, e = 0;
-      unsigned du = kInf;
-      i32x4 rec[kRec];
-      bool csr = false;
+    GDB_FOR_UNIFORM(t, valid, (int)threadIdx.x, (int)blockDim.x, ln * kChunk) {
+      // A wave without an entry skips the body: its instructions would take
+      // issue slots from the waves on its SIMD that have entries.
+      if (!__ballot(valid)) continue;
+      const int sl = t & (kChunk - 1);
+      int li = 0, b = 0, e = 0, v = kSlotNone;
+      unsigned du = kInf, w1 = 1u;
       if (valid) {
-        li = q[qa * qc + i];
+        li = q[qa * qc + (t >> 2)];
         // Clear the mark before reading the distance: a later improvement
-        // queues the vertex again.
-        // (A wave's LDS operations complete in order; the signal fence
-        // keeps the compiler from swapping the two.)
-        lds_and(&inq[li >> 5], ~(1u << (li & 31)));
+        // queues the vertex again.  (A vertex's four lanes share a wave,
+        // whose LDS operations complete in order; the signal fence keeps
+        // the compiler from swapping the two.)
+        if (sl == 0) lds_and(&inq[li >> 5], ~(1u << (li & 31)));
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         du = __hip_atomic_load(&dist[li], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-#pragma unroll
-        for (int h = 0; h < kRec; ++h) rec[h] = slots[kRec * (size_t)(lo + li) + h];
-      } else {
-#pragma unroll
-        for (int h = 0; h < kRec; ++h) rec[h] = i32x4{kSlotNone, 0, kSlotNone, 0};
-      }
-      if (valid) {
-        csr = rec[0].x == kSlotCsr;
-        if (csr) {
-          b = rowl ? rowl[li] : row[lo + li];
-          e = rowl ? rowl[li + 1] : row[lo + li + 1];
+        if (UNIT) {
+          v = ((g_ci32*)slots)[kChunk * (size_t)(lo + li) + sl];
+        } else {
+          const i32x2 p = ((g_ci32x2*)slots)[kChunk * (size_t)(lo + li) + sl];
+          v = p.x;
+          w1 = (unsigned)p.y;
         }
-        ++items;
+        if (sl == 0) ++items;
+      }
+      const bool csr = v == kSlotCsr;  // slot 0 of a vertex with more than kChunk edges
+      {
+        const bool ok = v != kSlotNone && !csr && w1 < kInf;
+        const bool loc = (unsigned)(v - lo) < (unsigned)nb;
+        const unsigned cand = du + w1;
+        if (ok && !loc && (unsigned)v < (unsigned)n)
+          __hip_atomic_fetch_min(&out[v], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool act[1] = {ok && loc};
+        const int lq[1] = {loc ? v - lo : 0};
+        const unsigned cd[1] = {cand};
+        bool imp[1];
+        relax_many(act, lq, cd, imp, qb, cb);
+        edges += v != kSlotNone && !csr;
+      }
+      if (!__ballot(csr)) continue;
+      if (csr) {
+        b = rowl ? rowl[li] : row[lo + li];
+        e = rowl ? rowl[li + 1] : row[lo + li + 1];
       }
       // Relaxes up to kChunk targets v (weights wt) of a vertex at src_d.
       auto relax_targets = [&](const bool (&ok)[kChunk], const int (&v)[kChunk], const unsigned (&wt)[kChunk],
@@ -1381,19 +1401,6 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
         }
         relax_many(act, li, cand, imp, qb, cb);
       };
-      {
-        bool ok[kChunk];
-        int v[kChunk];
-        unsigned wt[kChunk];
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-          v[j] = UNIT ? rec[0][j] : rec[j >> 1][2 * (j & 1)];
-          wt[j] = UNIT ? 1u : (unsigned)rec[j >> 1][2 * (j & 1) + 1];
-          ok[j] = valid && !csr && v[j] != kSlotNone;
-          edges += ok[j];
-        }
-        relax_targets(ok, v, wt, du);
-      }
       auto relax_edges = [&](const bool (&ok)[kChunk], const int (&kk)[kChunk], unsigned src_d) {
         int v[kChunk];
         unsigned wt[kChunk];
@@ -1496,7 +1503,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
       // A boundary scan every kScanEvery rounds.  When the queue leaves
       // waves without an entry, those waves scan while the others run the
       // round (the two overlap); otherwise the whole group scans after it.
-      const int split = (min(s_cnt[ca], qc) + 63) & ~63;  // threads with an entry, in whole 64-lane waves
+      const int split = (min(s_cnt[ca], qc) * kChunk + 63) & ~63;  // threads with an entry, in whole 64-lane waves
       const bool due = s % kScanEvery == kScanEvery - 1;
       if (due && split < (int)blockDim.x && (int)threadIdx.x >= split) {
         scan(qb, cb, (int)threadIdx.x - split, (int)blockDim.x - split);
