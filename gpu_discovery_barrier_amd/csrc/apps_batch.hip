@@ -1027,12 +1027,11 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
 // LDS (dynamic, `region` bytes): dist[B] u32 | inq[B/32] | far[B/32] |
 // queue u16[2][qcap] | row offsets[B + 1] when they fit | boundary list u16.
 // Rounds between boundary scans of a busy group, and BFS levels per band,
-// as measured on the 2048^2 grid (128 ranges; scans every 2 / 4 / 6 / 8
-// rounds, bands of 512 and 1024 levels; scans beside the rounds): 4 for both
-// (SSSP 12.8 ms at 4, 13.1 at 6, 13.3 at 8, 13.5+ at 2; BFS 8.65 ms at 4
-// with 1024-level bands, 8.8 with 512).  When the scan ran after the round
-// instead, 8 was best (SSSP 14.2 ms against 14.9 at 4).
-constexpr int kOwnedScanEveryBfs = 4, kOwnedScanEverySsspRows = 4, kOwnedScanEverySssp = 4;
+// as measured on the 2048^2 grid (128 ranges, four lanes per queued vertex,
+// scans beside the rounds when the spare waves cover the boundary in one
+// batch): 8 for both (BFS+SSSP 11.2 ms at 8, 11.4 at 6, 11.8 at 4; every 2
+// to 16 rounds measured); BFS bands of 1024 levels (2048: no better).
+constexpr int kOwnedScanEveryBfs = 8, kOwnedScanEverySsspRows = 8, kOwnedScanEverySssp = 8;
 constexpr int kOwnedBfsBand = 1024;
 // Queue entries per buffer at most: a larger tile would take the LDS the
 // boundary list needs (a range whose list does not fit scans all of its
@@ -1500,16 +1499,19 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
       if (s_cnt[ca] == 0) break;
       const int cb = (s + 1) % 3, qa = s & 1, qb = qa ^ 1;
       if (threadIdx.x == 0) s_cnt[(s + 2) % 3] = 0;
-      // A boundary scan every kScanEvery rounds.  When the queue leaves
-      // waves without an entry, those waves scan while the others run the
-      // round (the two overlap); otherwise the whole group scans after it.
+      // A boundary scan every kScanEvery rounds.  When the waves without a
+      // queue entry can load every boundary word in one batch, they scan
+      // while the others run the round (the two overlap); otherwise the
+      // whole group scans after it (a scan costs a far-memory latency per
+      // batch).
       const int split = (min(s_cnt[ca], qc) * kChunk + 63) & ~63;  // threads with an entry, in whole 64-lane waves
       const bool due = s % kScanEvery == kScanEvery - 1;
-      if (due && split < (int)blockDim.x && (int)threadIdx.x >= split) {
+      const bool beside = ((int)blockDim.x - split) * kScanBatch >= nscan;
+      if (due && beside && (int)threadIdx.x >= split) {
         scan(qb, cb, (int)threadIdx.x - split, (int)blockDim.x - split);
       } else {
         round(qa, ca, qb, cb);
-        if (due && split >= (int)blockDim.x) scan(qb, cb, (int)threadIdx.x, (int)blockDim.x);
+        if (due && !beside) scan(qb, cb, (int)threadIdx.x, (int)blockDim.x);
       }
       ++s;
       __syncthreads();
