@@ -1029,9 +1029,10 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
 // Rounds between boundary scans of a busy group, and BFS levels per band,
 // as measured on the 2048^2 grid (128 ranges, four lanes per queued vertex,
 // scans beside the rounds when the spare waves cover the boundary in one
-// batch): 8 for both (BFS+SSSP 11.2 ms at 8, 11.4 at 6, 11.8 at 4; every 2
-// to 16 rounds measured); BFS bands of 1024 levels (2048: no better).
-constexpr int kOwnedScanEveryBfs = 8, kOwnedScanEverySsspRows = 8, kOwnedScanEverySssp = 8;
+// batch): scans every 8 rounds for BFS and SSSP (BFS+SSSP 11.2 ms at 8, 11.4
+// at 6, 11.8 at 4; every 2 to 16 rounds measured); BFS bands of 1024 levels
+// (2048: no better).
+constexpr int kOwnedScanEvery = 8;
 constexpr int kOwnedBfsBand = 1024;
 // Queue entries per buffer at most: a larger tile would take the LDS the
 // boundary list needs (a range whose list does not fit scans all of its
@@ -1230,7 +1231,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
   const int nscan = scan_all ? nb : nbl;
 
   const unsigned band = UNIT ? (unsigned)kOwnedBfsBand : delta > 0 ? (unsigned)delta : kInf;
-  const int kScanEvery = UNIT ? kOwnedScanEveryBfs : rowl ? kOwnedScanEverySsspRows : kOwnedScanEverySssp;
+  constexpr int kScanEvery = kOwnedScanEvery;
   unsigned thresh = band;
   long long items = 0, edges = 0;
   // Relaxes local vertices li[j] to cand[j] on lanes with act[j], N targets
