@@ -1051,9 +1051,7 @@ constexpr int kSlotNone = -2147483647 - 1;
 constexpr int kSlotCsr = -2147483647;
 static_assert(kChunk == 4, "a slot record is one or two 16-byte vectors");
 typedef int i32x4 __attribute__((ext_vector_type(4)));
-typedef int i32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) i32x4 g_i32x4;
-typedef __attribute__((address_space(1))) const i32x2 g_ci32x2;
 // Scratch ints of an owned task: boundary marks, then the slot records.
 __host__ __device__ inline size_t owned_scratch_ints(int n, bool unit) {
   return ((((size_t)n + 31) / 32 + 63) & ~(size_t)63) + (size_t)n * kChunk * (unit ? 1 : 2);
@@ -1334,49 +1332,67 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
     if (__ballot(any) && lane == 0) s_flag = 1;
   };
   // Relaxes the out-edges of queue buffer (qa, count ca) into (qb, cb).
-  // Four lanes per queued vertex, one slot-record edge each: a round is
-  // usually run by one or two waves, so its length is their instruction
-  // count.  The first lane of a vertex marked kSlotCsr walks its CSR row.
+  // Several lanes per queued vertex, each relaxing kPer of its slot-record
+  // edges: a round is usually run by a few waves, so its length is their
+  // instruction count.  BFS: four lanes of one edge; SSSP: two lanes of two
+  // (one 16-byte load each; 10.9 against 11.2 ms with four lanes on the
+  // 2048^2 grid).  The first lane of a vertex marked kSlotCsr walks its CSR
+  // row.
+  constexpr int kLanes = UNIT ? 4 : 2, kPer = kChunk / kLanes;
   auto round = [&](int qa, int ca, int qb, int cb) {
     const int ln = min(s_cnt[ca], qc);
-    GDB_FOR_UNIFORM(t, valid, (int)threadIdx.x, (int)blockDim.x, ln * kChunk) {
+    GDB_FOR_UNIFORM(t, valid, (int)threadIdx.x, (int)blockDim.x, ln * kLanes) {
       // A wave without an entry skips the body: its instructions would take
       // issue slots from the waves on its SIMD that have entries.
       if (!__ballot(valid)) continue;
-      const int sl = t & (kChunk - 1);
-      int li = 0, b = 0, e = 0, v = kSlotNone;
-      unsigned du = kInf, w1 = 1u;
+      const int sl = t % kLanes;
+      int li = 0, b = 0, e = 0;
+      int v[kPer];
+      unsigned wt[kPer];
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        v[j] = kSlotNone;
+        wt[j] = 1u;
+      }
+      unsigned du = kInf;
       if (valid) {
-        li = q[qa * qc + (t >> 2)];
+        li = q[qa * qc + t / kLanes];
         // Clear the mark before reading the distance: a later improvement
-        // queues the vertex again.  (A vertex's four lanes share a wave,
-        // whose LDS operations complete in order; the signal fence keeps
-        // the compiler from swapping the two.)
+        // queues the vertex again.  (A vertex's lanes share a wave, whose
+        // LDS operations complete in order; the signal fence keeps the
+        // compiler from swapping the two.)
         if (sl == 0) lds_and(&inq[li >> 5], ~(1u << (li & 31)));
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         du = __hip_atomic_load(&dist[li], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (UNIT) {
-          v = ((g_ci32*)slots)[kChunk * (size_t)(lo + li) + sl];
+        if constexpr (UNIT) {
+          v[0] = ((g_ci32*)slots)[kChunk * (size_t)(lo + li) + sl];
         } else {
-          const i32x2 p = ((g_ci32x2*)slots)[kChunk * (size_t)(lo + li) + sl];
-          v = p.x;
-          w1 = (unsigned)p.y;
+          const i32x4 p = slots[2 * (size_t)(lo + li) + sl];  // two (id, weight) pairs
+          v[0] = p.x;
+          wt[0] = (unsigned)p.y;
+          v[1] = p.z;
+          wt[1] = (unsigned)p.w;
         }
         if (sl == 0) ++items;
       }
-      const bool csr = v == kSlotCsr;  // slot 0 of a vertex with more than kChunk edges
+      const bool csr = v[0] == kSlotCsr;  // slot 0 of a vertex with more than kChunk edges
       {
-        const bool ok = v != kSlotNone && !csr && w1 < kInf;
-        const bool loc = (unsigned)(v - lo) < (unsigned)nb;
-        const unsigned cand = du + w1;
-        if (ok && !loc && (unsigned)v < (unsigned)n)
-          __hip_atomic_fetch_min(&out[v], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const bool act[1] = {ok && loc};
-        const int lq[1] = {loc ? v - lo : 0};
-        const unsigned cd[1] = {cand};
-        bool imp[1];
+        bool act[kPer], imp[kPer];
+        int lq[kPer];
+        unsigned cd[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+          const bool edge = v[j] != kSlotNone && v[j] != kSlotCsr;
+          const bool ok = edge && wt[j] < kInf;
+          const bool loc = (unsigned)(v[j] - lo) < (unsigned)nb;
+          cd[j] = du + wt[j];
+          if (ok && !loc && (unsigned)v[j] < (unsigned)n)
+            __hip_atomic_fetch_min(&out[v[j]], cd[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          act[j] = ok && loc;
+          lq[j] = loc ? v[j] - lo : 0;
+          edges += edge;
+        }
         relax_many(act, lq, cd, imp, qb, cb);
-        edges += v != kSlotNone && !csr;
       }
       if (!__ballot(csr)) continue;
       if (csr) {
@@ -1505,7 +1521,7 @@ __device__ int team_owned(Team& T, const DevTask& k, unsigned long long* work, i
       // while the others run the round (the two overlap); otherwise the
       // whole group scans after it (a scan costs a far-memory latency per
       // batch).
-      const int split = (min(s_cnt[ca], qc) * kChunk + 63) & ~63;  // threads with an entry, in whole 64-lane waves
+      const int split = (min(s_cnt[ca], qc) * kLanes + 63) & ~63;  // threads with an entry, in whole 64-lane waves
       const bool due = s % kScanEvery == kScanEvery - 1;
       const bool beside = ((int)blockDim.x - split) * kScanBatch >= nscan;
       if (due && beside && (int)threadIdx.x >= split) {
