@@ -1030,7 +1030,8 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
 // as measured on the 2048^2 grid (128 ranges, four lanes per queued vertex,
 // scans beside the rounds when the spare waves cover the boundary in one
 // batch): scans every 8 rounds for BFS and SSSP (BFS+SSSP 11.2 ms at 8, 11.4
-// at 6, 11.8 at 4; every 2 to 16 rounds measured); BFS bands of 1024 levels
+// at 6, 11.8 at 4; every 2 to 16 rounds measured; with two lanes per SSSP
+// vertex, 10 rounds measured within 1 % of 8); BFS bands of 1024 levels
 // (2048: no better).
 constexpr int kOwnedScanEvery = 8;
 constexpr int kOwnedBfsBand = 1024;
