@@ -1029,11 +1029,12 @@ __device__ __forceinline__ int team_color(Team& T, const DevTask& k) {
 // Rounds between boundary scans of a busy group, and BFS levels per band,
 // as measured on the 2048^2 grid (128 ranges, four lanes per queued vertex,
 // scans beside the rounds when the spare waves cover the boundary in one
-// batch): scans every 8 rounds for BFS and SSSP (BFS+SSSP 11.2 ms at 8, 11.4
-// at 6, 11.8 at 4; every 2 to 16 rounds measured; with two lanes per SSSP
-// vertex, 10 rounds measured within 1 % of 8); BFS bands of 1024 levels
-// (2048: no better).
-constexpr int kOwnedScanEvery = 8;
+// batch): with four lanes per vertex, BFS+SSSP took 11.2 ms at 8 rounds, 11.4
+// at 6 and 11.8 at 4 (every 2 to 16 measured); with two lanes per SSSP
+// vertex, 10.80-10.83 ms at 10 rounds against 10.88-10.97 at 8, 10.87-10.93
+// at 12 and 10.95-10.96 at 6 (same box).  BFS bands of 1024 levels (2048: no
+// better).
+constexpr int kOwnedScanEvery = 10;
 constexpr int kOwnedBfsBand = 1024;
 // Queue entries per buffer at most: a larger tile would take the LDS the
 // boundary list needs (a range whose list does not fit scans all of its
